@@ -1,0 +1,444 @@
+"""Python binding of the MI355X BVH builder / traverser (libatrbvh.so, C ABI in
+include/atrbvh.h).
+
+The host-side names mirror the BVHRT-Core builder API so tests read like the
+reference's call sequence (gpu-ray-traversal/src/rt/cuda/Renderer.cpp:170-216):
+
+    ctx = Context(0)
+    scene = ctx.scene(vertices)                    # SceneWrapper
+    tree = LBVHBuilder(use64=True).BuildTree(scene) # LBVHBuilder::BuildTree
+    AgglomerativeTreeletOptimizer(9, 2).Optimize(tree)
+    collapsed, sah = BVHTreeCollapser().Collapse(tree)
+
+Every call goes to the gfx950 kernels; there is no CPU fallback.  Importing
+works without a GPU (the library loads); creating a Context needs one.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libatrbvh.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+
+ATR_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE", 4: "UNSUPPORTED",
+                5: "INTERNAL"}
+
+
+class AtrError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class TreeView(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint32), ("root", ctypes.c_int32),
+                ("parent", _i32p), ("left", _i32p), ("right", _i32p), ("data", _i32p),
+                ("bbmin", _f32p), ("bbmax", _f32p), ("area", _f32p)]
+
+
+class CollapsedView(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint32), ("root", ctypes.c_int32),
+                ("used", ctypes.c_uint32), ("parent", _i32p), ("left", _i32p),
+                ("right", _i32p), ("data", _i32p), ("triangle_count", _i32p),
+                ("bbmin", _f32p), ("bbmax", _f32p), ("triangle_indices", _i32p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libatrbvh.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with atrbvh_amd/build.py "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.atr_last_error.restype = ctypes.c_char_p
+        L.atr_version.restype = ctypes.c_char_p
+        L.atr_context_stream.restype = _vp
+        L.atr_context_stream.argtypes = [_vp]
+        L.atr_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+        for name in ("atr_profile_count", "atr_context_destroy", "atr_sync",
+                     "atr_profile_reset"):
+            getattr(L, name).argtypes = [_vp]
+        L.atr_profile_enable.argtypes = [_vp, ctypes.c_int]
+        L.atr_malloc.argtypes = [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.atr_free.argtypes = [_vp, _vp]
+        L.atr_memcpy_h2d.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.atr_memcpy_d2h.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.atr_memset_d32.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_size_t]
+        L.atr_profile_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                      ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.atr_scene_create.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, _vp, ctypes.POINTER(_vp)]
+        L.atr_scene_create_device.argtypes = L.atr_scene_create.argtypes
+        L.atr_scene_bounds.argtypes = [_vp, _f32p, _f32p]
+        L.atr_scene_vertices.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.atr_scene_destroy.argtypes = [_vp]
+        L.atr_lbvh_build.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.atr_trbvh_optimize.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                         ctypes.c_float]
+        L.atr_atrbvh_optimize.argtypes = L.atr_trbvh_optimize.argtypes
+        L.atr_collapse.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.POINTER(_vp),
+                                   _f32p]
+        L.atr_collapsed_stats.argtypes = [_vp, _f32p, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_uint32)]
+        L.atr_tree_sah.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _f32p]
+        L.atr_tree_num_triangles.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.atr_tree_download.argtypes = [_vp, ctypes.POINTER(TreeView)]
+        L.atr_tree_device_view.argtypes = [_vp, ctypes.POINTER(TreeView)]
+        L.atr_tree_upload.argtypes = [_vp, ctypes.POINTER(TreeView)]
+        L.atr_tree_create_from_host.argtypes = [_vp, ctypes.POINTER(TreeView),
+                                                ctypes.POINTER(_vp)]
+        L.atr_tree_bind_scene.argtypes = [_vp, _vp]
+        L.atr_tree_destroy.argtypes = [_vp]
+        L.atr_collapsed_download.argtypes = [_vp, ctypes.POINTER(CollapsedView)]
+        L.atr_collapsed_destroy.argtypes = [_vp]
+        L.atr_trace.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]
+        L.atr_rays_pinhole.argtypes = [_vp, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32,
+                                       ctypes.c_uint32, ctypes.c_float, _vp]
+        L.atr_rays_diffuse.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_float, _vp]
+        L.atr_morton_encode.argtypes = [_vp, _vp, ctypes.c_uint32, _f32p, _f32p, ctypes.c_int,
+                                        _vp, _vp]
+        L.atr_sort_pairs.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32]
+        L.atr_refit.argtypes = [_vp, _vp]
+        L.atr_gen_soup.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_float, _f32p]
+        L.atr_gen_soup.restype = None
+        L.atr_gen_gmm.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  _f32p]
+        L.atr_gen_gmm.restype = None
+        L.atr_trbvh_schedule.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != ATR_OK:
+        raise AtrError(status, lib().atr_last_error().decode())
+
+
+def _ptr(a: np.ndarray, t=_f32p):
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------- scenes (host)
+def gen_soup(seed: int, n: int, sigma: float = 0.005) -> np.ndarray:
+    v = np.zeros((3 * n, 4), np.float32)
+    lib().atr_gen_soup(seed, n, sigma, _ptr(v))
+    return v
+
+
+def gen_gmm(seed: int, n: int, clusters: int = 64, sigma_lo: float = 0.005, sigma_hi: float = 0.1,
+            edge_median: float = 0.002, edge_sigma: float = 0.7) -> np.ndarray:
+    v = np.zeros((3 * n, 4), np.float32)
+    lib().atr_gen_gmm(seed, n, clusters, sigma_lo, sigma_hi, edge_median, edge_sigma, _ptr(v))
+    return v
+
+
+def trbvh_schedule(treelet: int = 7, warp: int = 32) -> np.ndarray:
+    buf = np.zeros(4096, np.int32)
+    rounds = lib().atr_trbvh_schedule(treelet, warp, _ptr(buf, _i32p), buf.size)
+    if rounds < 0:
+        raise AtrError(1, "schedule generation failed")
+    return buf[: rounds * warp].reshape(rounds, warp)
+
+
+# ---------------------------------------------------------------- device objects
+class DeviceBuffer:
+    """Device memory from the context's caching allocator."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(lib().atr_malloc(ctx.h, max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().atr_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().atr_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().atr_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _vp()
+        check(lib().atr_context_create(device, ctypes.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            lib().atr_context_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(lib().atr_sync(self.h))
+
+    def buffer(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, a: np.ndarray) -> DeviceBuffer:
+        b = DeviceBuffer(self, a.nbytes)
+        b.upload(a)
+        return b
+
+    def scene(self, vertices: np.ndarray, bbmin=None, bbmax=None) -> "Scene":
+        return Scene(self, vertices, bbmin, bbmax)
+
+    # profiling
+    def profile(self, on: bool = True):
+        check(lib().atr_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        check(lib().atr_profile_reset(self.h))
+
+    def profile_records(self):
+        out = {}
+        for i in range(lib().atr_profile_count(self.h)):
+            name = ctypes.c_char_p()
+            launches = ctypes.c_int()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            check(lib().atr_profile_get(self.h, i, ctypes.byref(name), ctypes.byref(launches),
+                                        ctypes.byref(ms), ctypes.byref(by)))
+            out[name.value.decode()] = {"launches": launches.value, "ms": ms.value,
+                                        "bytes": by.value}
+        return out
+
+    # low-level stages (DeviceGenerateMortonCodes / DeviceSort)
+    def morton_encode(self, d_vertices: DeviceBuffer, n: int, bbmin, bbmax, bits: int,
+                      d_keys: DeviceBuffer, d_idx: DeviceBuffer | None):
+        mn = np.asarray(bbmin, np.float32)
+        mx = np.asarray(bbmax, np.float32)
+        check(lib().atr_morton_encode(self.h, d_vertices.ptr, n, _ptr(mn), _ptr(mx), bits,
+                                      d_keys.ptr, d_idx.ptr if d_idx is not None else None))
+
+    def sort_pairs(self, key_bits: int, d_keys: DeviceBuffer, d_vals: DeviceBuffer | None, n: int):
+        check(lib().atr_sort_pairs(self.h, key_bits, d_keys.ptr,
+                                   d_vals.ptr if d_vals is not None else None, n))
+
+    def rays_pinhole(self, eye, basis, tan_half: float, w: int, h: int, tmax: float,
+                     d_rays: DeviceBuffer):
+        e = np.asarray(eye, np.float32)
+        b = np.asarray(basis, np.float32).reshape(9)
+        check(lib().atr_rays_pinhole(self.h, _ptr(e), _ptr(b), tan_half, w, h, tmax, d_rays.ptr))
+
+    def rays_diffuse(self, scene: "Scene", d_primary: DeviceBuffer, d_results: DeviceBuffer,
+                     n: int, seed: int, tmax: float, d_rays: DeviceBuffer):
+        check(lib().atr_rays_diffuse(self.h, scene.h, d_primary.ptr, d_results.ptr, n, seed, tmax,
+                                     d_rays.ptr))
+
+
+class Scene:
+    """SceneWrapper (BVHRT-Core/src/SceneWrapper.cpp:8-30); vertices float4[3n]."""
+
+    def __init__(self, ctx: Context, vertices: np.ndarray, bbmin=None, bbmax=None):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 4)
+        assert v.shape[0] % 3 == 0
+        self.ctx = ctx
+        self.n = v.shape[0] // 3
+        h = _vp()
+        if bbmin is None:
+            check(lib().atr_scene_create(ctx.h, v.ctypes.data, self.n, None, None,
+                                         ctypes.byref(h)))
+        else:
+            mn = np.asarray(bbmin, np.float32)
+            mx = np.asarray(bbmax, np.float32)
+            check(lib().atr_scene_create(ctx.h, v.ctypes.data, self.n, mn.ctypes.data,
+                                         mx.ctypes.data, ctypes.byref(h)))
+        self.h = h.value
+
+    def bounds(self):
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        check(lib().atr_scene_bounds(self.h, _ptr(mn), _ptr(mx)))
+        return mn, mx
+
+    def device_vertices(self) -> int:
+        p = _vp()
+        check(lib().atr_scene_vertices(self.h, ctypes.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.h:
+            lib().atr_scene_destroy(self.h)
+            self.h = None
+
+
+class Tree:
+    """Device tree in the BVHRT SoA layout (SoABVHTree, BVHRT-Core/src/SoABVHTree.h)."""
+
+    def __init__(self, ctx: Context, handle: int, n: int):
+        self.ctx = ctx
+        self.h = handle
+        self.n = n
+
+    @classmethod
+    def from_host(cls, ctx: Context, host: dict, scene: Scene | None = None) -> "Tree":
+        n = int(host["n"])
+        arrs = {k: np.ascontiguousarray(host[k], np.int32) for k in ("parent", "left", "right",
+                                                                       "data")}
+        bmin = np.ascontiguousarray(host["bbmin"], np.float32).reshape(-1, 4)
+        bmax = np.ascontiguousarray(host["bbmax"], np.float32).reshape(-1, 4)
+        area = np.ascontiguousarray(host["area"], np.float32)
+        v = TreeView(n, 0, _ptr(arrs["parent"], _i32p), _ptr(arrs["left"], _i32p),
+                     _ptr(arrs["right"], _i32p), _ptr(arrs["data"], _i32p), _ptr(bmin),
+                     _ptr(bmax), _ptr(area))
+        h = _vp()
+        check(lib().atr_tree_create_from_host(ctx.h, ctypes.byref(v), ctypes.byref(h)))
+        t = cls(ctx, h.value, n)
+        if scene is not None:
+            check(lib().atr_tree_bind_scene(t.h, scene.h))
+        return t
+
+    def download(self) -> dict:
+        m = 2 * self.n - 1
+        out = {"n": self.n, "root": 0,
+               "parent": np.empty(m, np.int32), "left": np.empty(m, np.int32),
+               "right": np.empty(m, np.int32), "data": np.empty(m, np.int32),
+               "bbmin": np.empty((m, 4), np.float32), "bbmax": np.empty((m, 4), np.float32),
+               "area": np.empty(m, np.float32)}
+        v = TreeView(self.n, 0, _ptr(out["parent"], _i32p), _ptr(out["left"], _i32p),
+                     _ptr(out["right"], _i32p), _ptr(out["data"], _i32p), _ptr(out["bbmin"]),
+                     _ptr(out["bbmax"]), _ptr(out["area"]))
+        check(lib().atr_tree_download(self.h, ctypes.byref(v)))
+        return out
+
+    def SAH(self, ci: float = 1.2, ct: float = 1.0) -> float:
+        out = ctypes.c_float()
+        check(lib().atr_tree_sah(self.h, ci, ct, ctypes.byref(out)))
+        return out.value
+
+    def refit(self, scene: Scene | None = None):
+        check(lib().atr_refit(self.h, scene.h if scene is not None else None))
+
+    def close(self):
+        if self.h:
+            lib().atr_tree_destroy(self.h)
+            self.h = None
+
+
+class Collapsed:
+    """BVHCollapsedTree plus its traversal layout."""
+
+    def __init__(self, ctx: Context, handle: int, n: int):
+        self.ctx = ctx
+        self.h = handle
+        self.n = n
+
+    def stats(self):
+        sah = ctypes.c_float()
+        height = ctypes.c_int()
+        used = ctypes.c_uint32()
+        check(lib().atr_collapsed_stats(self.h, ctypes.byref(sah), ctypes.byref(height),
+                                        ctypes.byref(used)))
+        return {"sah": sah.value, "height": height.value, "used": used.value}
+
+    def download(self) -> dict:
+        m = 2 * self.n - 1
+        out = {"n": self.n, "root": 0,
+               "parent": np.empty(m, np.int32), "left": np.empty(m, np.int32),
+               "right": np.empty(m, np.int32), "data": np.empty(m, np.int32),
+               "count": np.empty(m, np.int32),
+               "bbmin": np.empty((m, 4), np.float32), "bbmax": np.empty((m, 4), np.float32),
+               "tri_index": np.empty(self.n, np.int32)}
+        v = CollapsedView(self.n, 0, 0, _ptr(out["parent"], _i32p), _ptr(out["left"], _i32p),
+                          _ptr(out["right"], _i32p), _ptr(out["data"], _i32p),
+                          _ptr(out["count"], _i32p), _ptr(out["bbmin"]), _ptr(out["bbmax"]),
+                          _ptr(out["tri_index"], _i32p))
+        check(lib().atr_collapsed_download(self.h, ctypes.byref(v)))
+        out["used"] = v.used
+        return out
+
+    def trace(self, d_rays: DeviceBuffer, n: int, d_results: DeviceBuffer, any_hit=False):
+        check(lib().atr_trace(self.h, d_rays.ptr, n, 1 if any_hit else 0, d_results.ptr))
+
+    def close(self):
+        if self.h:
+            lib().atr_collapsed_destroy(self.h)
+            self.h = None
+
+
+# ---------------------------------------------------------------- BVHRT mirror
+class LBVHBuilder:
+    """LBVHBuilder(bool use64BitMortonCodes) (BVHRT-Core/src/LBVHBuilder.h:31)."""
+
+    def __init__(self, use64BitMortonCodes: bool = False):
+        self.bits = 63 if use64BitMortonCodes else 30
+
+    def BuildTree(self, scene: Scene) -> Tree:
+        h = _vp()
+        check(lib().atr_lbvh_build(scene.h, self.bits, ctypes.byref(h)))
+        return Tree(scene.ctx, h.value, scene.n)
+
+
+class TRBVHOptimizer:
+    """TRBVHOptimizer(int treeletSize = 7, int iterations = 3) (TRBVHOptimizer.h:27)."""
+
+    def __init__(self, treeletSize: int = 7, iterations: int = 3, ci: float = 1.2,
+                 ct: float = 1.0):
+        self.treeletSize, self.iterations, self.ci, self.ct = treeletSize, iterations, ci, ct
+
+    def Optimize(self, tree: Tree):
+        check(lib().atr_trbvh_optimize(tree.h, self.treeletSize, self.iterations, self.ci,
+                                       self.ct))
+
+
+class AgglomerativeTreeletOptimizer:
+    """AgglomerativeTreeletOptimizer(int treeletSize = 9, int iterations = 2)
+    (AgglomerativeTreeletOptimizer.h:24)."""
+
+    def __init__(self, treeletSize: int = 9, iterations: int = 2, ci: float = 1.2,
+                 ct: float = 1.0):
+        self.treeletSize, self.iterations, self.ci, self.ct = treeletSize, iterations, ci, ct
+
+    def Optimize(self, tree: Tree):
+        check(lib().atr_atrbvh_optimize(tree.h, self.treeletSize, self.iterations, self.ci,
+                                        self.ct))
+
+
+class BVHTreeCollapser:
+    """BVHTreeCollapser::Collapse(tree, &sah, ci, ct) (BVHTreeCollapser.h:39)."""
+
+    def Collapse(self, tree: Tree, ci: float = 1.2, ct: float = 1.0, want_sah: bool = True):
+        h = _vp()
+        sah = ctypes.c_float()
+        check(lib().atr_collapse(tree.h, ci, ct, ctypes.byref(h),
+                                 ctypes.byref(sah) if want_sah else None))
+        return Collapsed(tree.ctx, h.value, tree.n), (sah.value if want_sah else None)

@@ -1,0 +1,859 @@
+// Host implementation of the C ABI in include/atrbvh.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace atr {
+
+namespace {
+thread_local std::string g_last_error;
+constexpr int kPinnedSlots = 4096;
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+atr_status fail(atr_status code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+// ------------------------------------------------------------- allocator
+atr_status dalloc(atr_context* ctx, size_t bytes, void** p)
+{
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~size_t(255);
+    auto it = ctx->free_blocks.find(bytes);
+    if (it != ctx->free_blocks.end()) {
+        *p = it->second;
+        ctx->free_blocks.erase(it);
+        ctx->live[*p] = bytes;
+        return ATR_OK;
+    }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        // release cached blocks and retry once
+        (void)hipGetLastError();
+        ATR_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
+        ctx->free_blocks.clear();
+        e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(ATR_ERR_OUT_OF_MEMORY,
+                        "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+    }
+    ctx->live[q] = bytes;
+    *p = q;
+    return ATR_OK;
+}
+
+// Returns a block to the cache.  Work on the context stream that still uses
+// it is ordered before any later user of the same block.
+void dfree(atr_context* ctx, void* p)
+{
+    if (!p) return;
+    auto it = ctx->live.find(p);
+    if (it == ctx->live.end()) return;
+    ctx->free_blocks.emplace(it->second, p);
+    ctx->live.erase(it);
+}
+
+// ------------------------------------------------------------- profiling
+static hipEvent_t take_event(atr_context* ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+LaunchScope::LaunchScope(atr_context* c, const char* name, double bytes) : ctx(c), rec(-1)
+{
+    if (!ctx->profiling) return;
+    auto it = ctx->prof_index.find(name);
+    if (it == ctx->prof_index.end()) {
+        ctx->prof.emplace_back();
+        ctx->prof.back().name = name;
+        rec = (int)ctx->prof.size() - 1;
+        ctx->prof_index[name] = rec;
+    } else {
+        rec = it->second;
+    }
+    ctx->prof[rec].bytes += bytes;
+    ctx->prof[rec].launches += 1;
+    e0 = take_event(ctx);
+    e1 = take_event(ctx);
+    if (e0) (void)hipEventRecord(e0, ctx->stream);
+}
+
+atr_status LaunchScope::finish()
+{
+    hipError_t e = hipGetLastError();
+    if (rec >= 0 && e0 && e1) {
+        (void)hipEventRecord(e1, ctx->stream);
+        ctx->prof[rec].pending.emplace_back(e0, e1);
+    }
+    if (e != hipSuccess)
+        return fail(ATR_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return ATR_OK;
+}
+
+static void resolve_profile(atr_context* ctx)
+{
+    for (auto& r : ctx->prof) {
+        for (auto& pr : r.pending) {
+            float ms = 0.0f;
+            (void)hipEventSynchronize(pr.second);
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) r.total_ms += ms;
+            ctx->event_pool.push_back(pr.first);
+            ctx->event_pool.push_back(pr.second);
+        }
+        r.pending.clear();
+    }
+}
+
+}  // namespace atr
+
+using namespace atr;
+
+// ================================================================== C ABI
+extern "C" {
+
+const char* atr_last_error(void) { return g_last_error.c_str(); }
+
+const char* atr_version(void) { return "atrbvh-mi355x 0.1 (gfx950)"; }
+
+atr_status atr_context_create(int device, atr_context** out)
+{
+    if (!out) return fail(ATR_ERR_INVALID_ARGUMENT, "out is NULL");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return fail(ATR_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return fail(ATR_ERR_INVALID_ARGUMENT, "bad device index");
+    ATR_HIP(hipSetDevice(device));
+    atr_context* ctx = new atr_context();
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(ATR_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    if (dalloc_t(ctx, 64, &ctx->d_status) != ATR_OK ||
+        hipMemsetAsync(ctx->d_status, 0, 256, ctx->stream) != hipSuccess ||
+        hipHostMalloc(&ctx->h_pinned, kPinnedSlots * sizeof(CollapseSummary)) != hipSuccess) {
+        atr_context_destroy(ctx);
+        return fail(ATR_ERR_DEVICE, "context setup failed");
+    }
+    for (int i = kPinnedSlots - 1; i >= 0; --i) ctx->pinned_free.push_back(i);
+    *out = ctx;
+    return ATR_OK;
+}
+
+atr_status atr_context_destroy(atr_context* ctx)
+{
+    if (!ctx) return ATR_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    resolve_profile(ctx);
+    for (auto& kv : ctx->live) (void)hipFree(kv.first);
+    for (auto& kv : ctx->free_blocks) (void)hipFree(kv.second);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ATR_OK;
+}
+
+atr_status atr_sync(atr_context* ctx)
+{
+    if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    ATR_HIP(hipStreamSynchronize(ctx->stream));
+    uint32_t st = 0;
+    ATR_HIP(hipMemcpy(&st, ctx->d_status, 4, hipMemcpyDeviceToHost));
+    if (st != 0) return fail(ATR_ERR_INTERNAL, "a bounded device wait gave up");
+    return ATR_OK;
+}
+
+void* atr_context_stream(atr_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+atr_status atr_malloc(atr_context* ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    return dalloc(ctx, bytes, dptr);
+}
+
+atr_status atr_free(atr_context* ctx, void* dptr)
+{
+    if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    dfree(ctx, dptr);
+    return ATR_OK;
+}
+
+atr_status atr_memcpy_h2d(atr_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL");
+    if (bytes == 0) return ATR_OK;
+    ATR_HIP(hipStreamSynchronize(ctx->stream));
+    ATR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ATR_OK;
+}
+
+atr_status atr_memcpy_d2h(atr_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL");
+    if (bytes == 0) return ATR_OK;
+    ATR_HIP(hipStreamSynchronize(ctx->stream));
+    ATR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ATR_OK;
+}
+
+atr_status atr_memset_d32(atr_context* ctx, void* dst, uint32_t value, size_t count)
+{
+    if (!ctx || (!dst && count)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL");
+    if (count == 0) return ATR_OK;
+    ATR_HIP(hipMemsetD32Async((hipDeviceptr_t)dst, (int)value, count, ctx->stream));
+    return ATR_OK;
+}
+
+atr_status atr_profile_enable(atr_context* ctx, int on)
+{
+    if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    ctx->profiling = on != 0;
+    return ATR_OK;
+}
+
+atr_status atr_profile_reset(atr_context* ctx)
+{
+    if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    resolve_profile(ctx);
+    ctx->prof.clear();
+    ctx->prof_index.clear();
+    return ATR_OK;
+}
+
+int atr_profile_count(atr_context* ctx) { return ctx ? (int)ctx->prof.size() : 0; }
+
+atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* launches,
+                           double* total_ms, double* bytes)
+{
+    if (!ctx || i < 0 || i >= (int)ctx->prof.size())
+        return fail(ATR_ERR_INVALID_ARGUMENT, "bad profile index");
+    resolve_profile(ctx);
+    const ProfileRecord& r = ctx->prof[i];
+    if (name) *name = r.name.c_str();
+    if (launches) *launches = r.launches;
+    if (total_ms) *total_ms = r.total_ms;
+    if (bytes) *bytes = r.bytes;
+    return ATR_OK;
+}
+
+// ------------------------------------------------------------------ scene
+static atr_status scene_finish(atr_context* ctx, atr_scene* s, const float* bbmin,
+                               const float* bbmax)
+{
+    ATR_TRY(dalloc_t(ctx, 8, &s->d_bbox));
+    if (bbmin && bbmax) {
+        for (int a = 0; a < 3; ++a) {
+            s->h_bbox[a] = bbmin[a];
+            s->h_bbox[3 + a] = bbmax[a];
+        }
+        s->bbox_on_host = true;
+        ATR_HIP(hipMemcpyAsync(s->d_bbox, s->h_bbox, 6 * sizeof(float), hipMemcpyHostToDevice,
+                               ctx->stream));
+        ATR_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        ATR_TRY(run_scene_bbox(ctx, s->verts, s->n, s->d_bbox));
+    }
+    return ATR_OK;
+}
+
+atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t n,
+                            const float* bbmin, const float* bbmax, atr_scene** out)
+{
+    if (!ctx || !vertices || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a scene needs at least 2 triangles");
+    if ((bbmin == nullptr) != (bbmax == nullptr))
+        return fail(ATR_ERR_INVALID_ARGUMENT, "bbmin and bbmax must both be given or both NULL");
+    for (size_t i = 0; i < 3ull * n; ++i)
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(vertices[i * 4 + a]))
+                return fail(ATR_ERR_INVALID_ARGUMENT,
+                            "non-finite vertex coordinate at vertex " + std::to_string(i));
+    atr_scene* s = new atr_scene();
+    s->ctx = ctx;
+    s->n = n;
+    atr_status st = dalloc_t(ctx, 3ull * n, &s->verts);
+    if (st != ATR_OK) {
+        delete s;
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(s->verts, vertices, 3ull * n * sizeof(float4),
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        atr_scene_destroy(s);
+        return fail(ATR_ERR_DEVICE, std::string("vertex upload: ") + hipGetErrorString(e));
+    }
+    st = scene_finish(ctx, s, bbmin, bbmax);
+    if (st != ATR_OK) {
+        atr_scene_destroy(s);
+        return st;
+    }
+    *out = s;
+    return ATR_OK;
+}
+
+atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices, uint32_t n,
+                                   const float* bbmin, const float* bbmax, atr_scene** out)
+{
+    if (!ctx || !d_vertices || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a scene needs at least 2 triangles");
+    if ((bbmin == nullptr) != (bbmax == nullptr))
+        return fail(ATR_ERR_INVALID_ARGUMENT, "bbmin and bbmax must both be given or both NULL");
+    atr_scene* s = new atr_scene();
+    s->ctx = ctx;
+    s->n = n;
+    atr_status st = dalloc_t(ctx, 3ull * n, &s->verts);
+    if (st != ATR_OK) {
+        delete s;
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(s->verts, d_vertices, 3ull * n * sizeof(float4),
+                                  hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        atr_scene_destroy(s);
+        return fail(ATR_ERR_DEVICE, std::string("vertex copy: ") + hipGetErrorString(e));
+    }
+    st = scene_finish(ctx, s, bbmin, bbmax);
+    if (st != ATR_OK) {
+        atr_scene_destroy(s);
+        return st;
+    }
+    *out = s;
+    return ATR_OK;
+}
+
+atr_status atr_scene_bounds(atr_scene* s, float bbmin[3], float bbmax[3])
+{
+    if (!s || !bbmin || !bbmax) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!s->bbox_on_host) {
+        ATR_HIP(hipStreamSynchronize(s->ctx->stream));
+        ATR_HIP(hipMemcpy(s->h_bbox, s->d_bbox, 6 * sizeof(float), hipMemcpyDeviceToHost));
+        s->bbox_on_host = true;
+    }
+    for (int a = 0; a < 3; ++a) {
+        bbmin[a] = s->h_bbox[a];
+        bbmax[a] = s->h_bbox[3 + a];
+    }
+    return ATR_OK;
+}
+
+atr_status atr_scene_vertices(atr_scene* s, float** d_vertices)
+{
+    if (!s || !d_vertices) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *d_vertices = (float*)s->verts;
+    return ATR_OK;
+}
+
+atr_status atr_scene_destroy(atr_scene* s)
+{
+    if (!s) return ATR_OK;
+    dfree(s->ctx, s->verts);
+    dfree(s->ctx, s->d_bbox);
+    delete s;
+    return ATR_OK;
+}
+
+// ------------------------------------------------------------------ trees
+static atr_status tree_alloc(atr_context* ctx, uint32_t n, atr_tree** out)
+{
+    atr_tree* t = new atr_tree();
+    t->ctx = ctx;
+    t->n = n;
+    const size_t m = 2ull * n - 1;
+    atr_status s = ATR_OK;
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->parent);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->left);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->right);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->data);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->bbmin);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->bbmax);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->area);
+    if (s != ATR_OK) {
+        atr_tree_destroy(t);
+        return s;
+    }
+    *out = t;
+    return ATR_OK;
+}
+
+atr_status atr_tree_destroy(atr_tree* t)
+{
+    if (!t) return ATR_OK;
+    atr_context* ctx = t->ctx;
+    dfree(ctx, t->parent);
+    dfree(ctx, t->left);
+    dfree(ctx, t->right);
+    dfree(ctx, t->data);
+    dfree(ctx, t->bbmin);
+    dfree(ctx, t->bbmax);
+    dfree(ctx, t->area);
+    delete t;
+    return ATR_OK;
+}
+
+atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out)
+{
+    if (!scene || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
+    atr_context* ctx = scene->ctx;
+    const uint32_t n = scene->n;
+    const int kb = morton_bits == 63 ? 8 : 4;
+    const int passes = morton_bits == 63 ? 8 : 4;
+    atr_tree* t = nullptr;
+    ATR_TRY(tree_alloc(ctx, n, &t));
+    void* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;
+    uint64_t* slots = nullptr;
+    const uint32_t blocks = std::max(encode_blocks(n), 1u);
+    atr_status s = ATR_OK;
+    if (s == ATR_OK) s = dalloc(ctx, (size_t)n * kb, &keys[0]);
+    if (s == ATR_OK) s = dalloc(ctx, (size_t)n * kb, &keys[1]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &vals[0]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &vals[1]);
+    if (s == ATR_OK) s = dalloc_t(ctx, (size_t)blocks * passes * 256, &hist);
+    if (s == ATR_OK) s = dalloc_t(ctx, n - 1, &slots);
+    if (s == ATR_OK)
+        s = run_encode(ctx, scene->verts, n, scene->d_bbox, morton_bits, keys[0], nullptr, hist,
+                       blocks, slots, n - 1, nullptr, 0);
+    bool in_alt = false;
+    if (s == ATR_OK)
+        s = run_sort(ctx, kb * 8, keys[0], vals[0], n, keys[1], vals[1], true, hist, blocks,
+                     &in_alt);
+    if (s == ATR_OK)
+        s = run_build(ctx, morton_bits, keys[in_alt ? 1 : 0], vals[in_alt ? 1 : 0], n,
+                      scene->verts, t, slots);
+    dfree(ctx, keys[0]);
+    dfree(ctx, keys[1]);
+    dfree(ctx, vals[0]);
+    dfree(ctx, vals[1]);
+    dfree(ctx, hist);
+    dfree(ctx, slots);
+    if (s != ATR_OK) {
+        atr_tree_destroy(t);
+        return s;
+    }
+    t->scene = scene;
+    *out = t;
+    return ATR_OK;
+}
+
+atr_status atr_tree_bind_scene(atr_tree* t, atr_scene* scene)
+{
+    if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
+    if (scene && scene->n != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "size mismatch");
+    t->scene = scene;
+    return ATR_OK;
+}
+
+static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int iterations, float ci,
+                                   float ct)
+{
+    atr_context* ctx = t->ctx;
+    const uint32_t n = t->n;
+    int32_t* tris = nullptr;
+    float* nsah = nullptr;
+    uint32_t* counters = nullptr;
+    int32_t* d_sched = nullptr;
+    int rounds = 0;
+    atr_status s = ATR_OK;
+    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n - 1, &tris);
+    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n - 1, &nsah);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &counters);
+    if (s == ATR_OK && !agg) {
+        std::vector<int32_t> sched;
+        rounds = trbvh_schedule(treelet, 32, sched);
+        if (rounds <= 0) s = fail(ATR_ERR_INTERNAL, "TRBVH schedule generation failed");
+        if (s == ATR_OK) s = dalloc_t(ctx, sched.size(), &d_sched);
+        if (s == ATR_OK) {
+            hipError_t e = hipMemcpyAsync(d_sched, sched.data(), sched.size() * 4,
+                                          hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "schedule upload failed");
+        }
+    }
+    int gamma = treelet;
+    for (int it = 0; it < iterations && s == ATR_OK; ++it) {
+        if (agg)
+            s = run_atrbvh_pass(ctx, t, treelet, gamma, ci, ct, tris, nsah, counters);
+        else
+            s = run_trbvh_pass(ctx, t, gamma, ci, ct, tris, nsah, counters, d_sched, rounds);
+        gamma *= 2;
+    }
+    dfree(ctx, tris);
+    dfree(ctx, nsah);
+    dfree(ctx, counters);
+    dfree(ctx, d_sched);
+    return s;
+}
+
+atr_status atr_trbvh_optimize(atr_tree* t, int treelet_size, int iterations, float ci, float ct)
+{
+    if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
+    if (treelet_size != 7)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "TRBVH supports treelet size 7 only");
+    if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    return treelet_optimize(t, false, treelet_size, iterations, ci, ct);
+}
+
+atr_status atr_atrbvh_optimize(atr_tree* t, int treelet_size, int iterations, float ci, float ct)
+{
+    if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
+    if (treelet_size < 3 || treelet_size > 32)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
+    if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    return treelet_optimize(t, true, treelet_size, iterations, ci, ct);
+}
+
+atr_status atr_tree_num_triangles(const atr_tree* t, uint32_t* n)
+{
+    if (!t || !n) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *n = t->n;
+    return ATR_OK;
+}
+
+atr_status atr_tree_device_view(atr_tree* t, atr_tree_view* v)
+{
+    if (!t || !v) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    v->num_triangles = t->n;
+    v->root = 0;
+    v->parent = t->parent;
+    v->left = t->left;
+    v->right = t->right;
+    v->data = t->data;
+    v->bbmin = (float*)t->bbmin;
+    v->bbmax = (float*)t->bbmax;
+    v->area = t->area;
+    return ATR_OK;
+}
+
+atr_status atr_tree_download(atr_tree* t, atr_tree_view* h)
+{
+    if (!t || !h) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t m = 2ull * t->n - 1;
+    ATR_HIP(hipStreamSynchronize(t->ctx->stream));
+    if (h->parent) ATR_HIP(hipMemcpy(h->parent, t->parent, m * 4, hipMemcpyDeviceToHost));
+    if (h->left) ATR_HIP(hipMemcpy(h->left, t->left, m * 4, hipMemcpyDeviceToHost));
+    if (h->right) ATR_HIP(hipMemcpy(h->right, t->right, m * 4, hipMemcpyDeviceToHost));
+    if (h->data) ATR_HIP(hipMemcpy(h->data, t->data, m * 4, hipMemcpyDeviceToHost));
+    if (h->bbmin) ATR_HIP(hipMemcpy(h->bbmin, t->bbmin, m * 16, hipMemcpyDeviceToHost));
+    if (h->bbmax) ATR_HIP(hipMemcpy(h->bbmax, t->bbmax, m * 16, hipMemcpyDeviceToHost));
+    if (h->area) ATR_HIP(hipMemcpy(h->area, t->area, m * 4, hipMemcpyDeviceToHost));
+    h->num_triangles = t->n;
+    h->root = 0;
+    return ATR_OK;
+}
+
+atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
+{
+    if (!t || !h || !h->parent || !h->left || !h->right || !h->data || !h->bbmin || !h->bbmax ||
+        !h->area)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (h->num_triangles != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "size mismatch");
+    const size_t m = 2ull * t->n - 1;
+    ATR_HIP(hipStreamSynchronize(t->ctx->stream));
+    ATR_HIP(hipMemcpy(t->parent, h->parent, m * 4, hipMemcpyHostToDevice));
+    ATR_HIP(hipMemcpy(t->left, h->left, m * 4, hipMemcpyHostToDevice));
+    ATR_HIP(hipMemcpy(t->right, h->right, m * 4, hipMemcpyHostToDevice));
+    ATR_HIP(hipMemcpy(t->data, h->data, m * 4, hipMemcpyHostToDevice));
+    ATR_HIP(hipMemcpy(t->bbmin, h->bbmin, m * 16, hipMemcpyHostToDevice));
+    ATR_HIP(hipMemcpy(t->bbmax, h->bbmax, m * 16, hipMemcpyHostToDevice));
+    ATR_HIP(hipMemcpy(t->area, h->area, m * 4, hipMemcpyHostToDevice));
+    return ATR_OK;
+}
+
+// Tree created from a host view (for tests and for callers holding a tree).
+atr_status atr_tree_create_from_host(atr_context* ctx, const atr_tree_view* h, atr_tree** out)
+{
+    if (!ctx || !h || !out || h->num_triangles < 2)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "bad argument");
+    atr_tree* t = nullptr;
+    ATR_TRY(tree_alloc(ctx, h->num_triangles, &t));
+    atr_status s = atr_tree_upload(t, h);
+    if (s != ATR_OK) {
+        atr_tree_destroy(t);
+        return s;
+    }
+    *out = t;
+    return ATR_OK;
+}
+
+atr_status atr_tree_sah(atr_tree* t, float ci, float ct, float* out)
+{
+    if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t m = 2ull * t->n - 1;
+    std::vector<float> a(m);
+    ATR_HIP(hipStreamSynchronize(t->ctx->stream));
+    ATR_HIP(hipMemcpy(a.data(), t->area, m * 4, hipMemcpyDeviceToHost));
+    // SoABVHTree::SAH order: root first, other internal nodes, then leaves
+    const uint32_t n = t->n;
+    const float total = a[0];
+    float si = total;
+    float sl = 0.0f;
+    for (uint32_t i = 1; i < n - 1; ++i) si += a[i];
+    for (uint32_t i = n - 1; i < 2 * n - 1; ++i) sl += a[i];
+    *out = (ci * si + ct * sl) / total;
+    return ATR_OK;
+}
+
+// ------------------------------------------------------------------ collapse
+static atr_status collapsed_alloc(atr_context* ctx, uint32_t n, atr_collapsed** out)
+{
+    atr_collapsed* c = new atr_collapsed();
+    c->ctx = ctx;
+    c->n = n;
+    const size_t m = 2ull * n - 1;
+    atr_status s = ATR_OK;
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->parent);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->left);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->right);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->data);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->count);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->bbmin);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->bbmax);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &c->tri_index);
+    if (s == ATR_OK) s = dalloc_t(ctx, 4ull * (n - 1), &c->tnodes);
+    if (s == ATR_OK) s = dalloc_t(ctx, 6ull * n, &c->ttris);
+    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &c->ttri_id);
+    if (s == ATR_OK) s = dalloc_t(ctx, 1, &c->d_summary);
+    if (s == ATR_OK) {
+        c->summary_event = take_event(ctx);
+        if (!c->summary_event) s = fail(ATR_ERR_DEVICE, "event creation failed");
+    }
+    if (s == ATR_OK) {
+        if (ctx->pinned_free.empty()) {
+            s = fail(ATR_ERR_OUT_OF_MEMORY, "too many live collapsed trees in this context");
+        } else {
+            c->pinned_slot = ctx->pinned_free.back();
+            ctx->pinned_free.pop_back();
+        }
+    }
+    if (s != ATR_OK) {
+        atr_collapsed_destroy(c);
+        return s;
+    }
+    *out = c;
+    return ATR_OK;
+}
+
+atr_status atr_collapsed_destroy(atr_collapsed* c)
+{
+    if (!c) return ATR_OK;
+    atr_context* ctx = c->ctx;
+    dfree(ctx, c->parent);
+    dfree(ctx, c->left);
+    dfree(ctx, c->right);
+    dfree(ctx, c->data);
+    dfree(ctx, c->count);
+    dfree(ctx, c->bbmin);
+    dfree(ctx, c->bbmax);
+    dfree(ctx, c->tri_index);
+    dfree(ctx, c->tnodes);
+    dfree(ctx, c->ttris);
+    dfree(ctx, c->ttri_id);
+    dfree(ctx, c->d_summary);
+    if (c->summary_event) {
+        (void)hipEventSynchronize(c->summary_event);
+        ctx->event_pool.push_back(c->summary_event);
+    }
+    if (c->pinned_slot >= 0) ctx->pinned_free.push_back(c->pinned_slot);
+    delete c;
+    return ATR_OK;
+}
+
+static atr_status collapsed_summary(atr_collapsed* c)
+{
+    if (c->summary_ready) return ATR_OK;
+    ATR_HIP(hipEventSynchronize(c->summary_event));
+    c->h_summary = ((const CollapseSummary*)c->ctx->h_pinned)[c->pinned_slot];
+    c->summary_ready = true;
+    return ATR_OK;
+}
+
+atr_status atr_collapse(atr_tree* t, float ci, float ct, atr_collapsed** out, float* sah)
+{
+    if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    atr_context* ctx = t->ctx;
+    const float4* verts = t->scene ? t->scene->verts : nullptr;
+    atr_collapsed* c = nullptr;
+    ATR_TRY(collapsed_alloc(ctx, t->n, &c));
+    c->has_layout = verts != nullptr;
+    atr_status s = run_collapse(ctx, t, c, ci, ct, verts);
+    if (s == ATR_OK) {
+        CollapseSummary* dst = (CollapseSummary*)ctx->h_pinned + c->pinned_slot;
+        hipError_t e = hipMemcpyAsync(dst, c->d_summary, sizeof(CollapseSummary),
+                                      hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->summary_event, ctx->stream);
+        if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "summary readback failed");
+    }
+    if (s == ATR_OK && sah) {
+        s = collapsed_summary(c);
+        if (s == ATR_OK) *sah = c->h_summary.cost_root / c->h_summary.area_root;
+    }
+    if (s != ATR_OK) {
+        atr_collapsed_destroy(c);
+        return s;
+    }
+    *out = c;
+    return ATR_OK;
+}
+
+atr_status atr_collapsed_stats(atr_collapsed* c, float* sah, int* height, uint32_t* used)
+{
+    if (!c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    ATR_TRY(collapsed_summary(c));
+    if (sah) *sah = c->h_summary.cost_root / c->h_summary.area_root;
+    if (height) *height = c->h_summary.height;
+    if (used) *used = c->h_summary.used;
+    return ATR_OK;
+}
+
+atr_status atr_collapsed_download(atr_collapsed* c, atr_collapsed_view* h)
+{
+    if (!c || !h) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    ATR_TRY(collapsed_summary(c));
+    const size_t m = 2ull * c->n - 1;
+    ATR_HIP(hipStreamSynchronize(c->ctx->stream));
+    if (h->parent) ATR_HIP(hipMemcpy(h->parent, c->parent, m * 4, hipMemcpyDeviceToHost));
+    if (h->left) ATR_HIP(hipMemcpy(h->left, c->left, m * 4, hipMemcpyDeviceToHost));
+    if (h->right) ATR_HIP(hipMemcpy(h->right, c->right, m * 4, hipMemcpyDeviceToHost));
+    if (h->data) ATR_HIP(hipMemcpy(h->data, c->data, m * 4, hipMemcpyDeviceToHost));
+    if (h->triangle_count)
+        ATR_HIP(hipMemcpy(h->triangle_count, c->count, m * 4, hipMemcpyDeviceToHost));
+    if (h->bbmin) ATR_HIP(hipMemcpy(h->bbmin, c->bbmin, m * 16, hipMemcpyDeviceToHost));
+    if (h->bbmax) ATR_HIP(hipMemcpy(h->bbmax, c->bbmax, m * 16, hipMemcpyDeviceToHost));
+    if (h->triangle_indices)
+        ATR_HIP(hipMemcpy(h->triangle_indices, c->tri_index, (size_t)c->n * 4,
+                          hipMemcpyDeviceToHost));
+    h->num_triangles = c->n;
+    h->root = 0;
+    h->used = c->h_summary.used;
+    return ATR_OK;
+}
+
+// ------------------------------------------------------------------ trace
+atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, int any_hit,
+                     int32_t* d_results)
+{
+    if (!c || (num_rays && (!d_rays || !d_results)))
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->has_layout)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "collapsed tree has no traversal layout: its tree was not bound to a scene");
+    ATR_TRY(collapsed_summary(c));
+    return run_trace(c->ctx, c, (const float4*)d_rays, num_rays, any_hit, d_results,
+                     c->h_summary.height);
+}
+
+atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float basis[9],
+                            float tan_half_fov, uint32_t width, uint32_t height, float tmax,
+                            float* d_rays)
+{
+    if (!ctx || !eye || !basis || !d_rays) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    return run_rays_pinhole(ctx, eye, basis, tan_half_fov, width, height, tmax, (float4*)d_rays);
+}
+
+atr_status atr_rays_diffuse(atr_context* ctx, atr_scene* scene, const float* d_primary,
+                            const int32_t* d_primary_results, uint32_t num_rays, uint32_t seed,
+                            float tmax, float* d_rays)
+{
+    if (!ctx || !scene || !d_primary || !d_primary_results || !d_rays)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    return run_rays_diffuse(ctx, scene->verts, (const float4*)d_primary, d_primary_results,
+                            num_rays, seed, tmax, (float4*)d_rays);
+}
+
+// ------------------------------------------------------------------ stages
+atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t n,
+                             const float bbmin[3], const float bbmax[3], int morton_bits,
+                             void* d_keys, uint32_t* d_indices)
+{
+    if (!ctx || !d_vertices || !bbmin || !bbmax || !d_keys)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
+    if (n == 0) return ATR_OK;
+    float* d_bbox = nullptr;
+    ATR_TRY(dalloc_t(ctx, 8, &d_bbox));
+    float hb[6] = {bbmin[0], bbmin[1], bbmin[2], bbmax[0], bbmax[1], bbmax[2]};
+    ATR_HIP(hipMemcpyAsync(d_bbox, hb, sizeof(hb), hipMemcpyHostToDevice, ctx->stream));
+    ATR_HIP(hipStreamSynchronize(ctx->stream));
+    atr_status s = run_encode(ctx, (const float4*)d_vertices, n, d_bbox, morton_bits, d_keys,
+                              d_indices, nullptr, 0, nullptr, 0, nullptr, 0);
+    dfree(ctx, d_bbox);
+    return s;
+}
+
+atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t* d_values,
+                          uint32_t n)
+{
+    if (!ctx || (n && !d_keys)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (key_bits != 32 && key_bits != 64)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "key_bits must be 32 or 64");
+    if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 keys per sort");
+    if (n <= 1) return ATR_OK;
+    void* kalt = nullptr;
+    uint32_t* valt = nullptr;
+    uint32_t* vtmp = nullptr;
+    ATR_TRY(dalloc(ctx, (size_t)n * key_bits / 8, &kalt));
+    ATR_TRY(dalloc_t(ctx, n, &valt));
+    const bool identity = d_values == nullptr;
+    if (identity) ATR_TRY(dalloc_t(ctx, n, &vtmp));
+    bool in_alt = false;
+    atr_status s = run_sort(ctx, key_bits, d_keys, identity ? vtmp : d_values, n, kalt, valt,
+                            identity, nullptr, 0, &in_alt);
+    if (s == ATR_OK && in_alt) {
+        hipError_t e = hipMemcpyAsync(d_keys, kalt, (size_t)n * key_bits / 8,
+                                      hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess && !identity)
+            e = hipMemcpyAsync(d_values, valt, (size_t)n * 4, hipMemcpyDeviceToDevice,
+                               ctx->stream);
+        if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "sort copy-back failed");
+    }
+    dfree(ctx, kalt);
+    dfree(ctx, valt);
+    dfree(ctx, vtmp);
+    return s;
+}
+
+atr_status atr_refit(atr_tree* t, atr_scene* scene)
+{
+    if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!scene) scene = t->scene;
+    if (!scene) return fail(ATR_ERR_INVALID_ARGUMENT, "no scene given or bound to the tree");
+    if (scene->n != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "scene/tree size mismatch");
+    uint32_t* counters = nullptr;
+    ATR_TRY(dalloc_t(t->ctx, t->n, &counters));
+    atr_status s = run_refit(t->ctx, t, scene->verts, counters);
+    dfree(t->ctx, counters);
+    return s;
+}
+
+}  // extern "C"

@@ -1,0 +1,360 @@
+// SAH leaf collapse (BVHRT-Core/src/TreeCollapser.cu) and the traversal
+// layout emitted from the collapsed tree on the GPU.
+//
+// Collapse semantics kept from the reference: a node P hands its children's
+// triangles to leaf lists iff the climbs from both of its children reach it
+// (every node strictly below on both sides is collapsible) and P itself is
+// not collapsible (collapseTreeKernel :173-237).  The reference takes list
+// offsets from an atomicAdd in arbitrary order; here they come from an
+// exclusive scan over emitting parents in node order (left list first), so
+// the output is deterministic.
+//
+// Traversal layout (this build's, consumed by trace.hip):
+//   tnodes[4p+0] = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   tnodes[4p+1] = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   tnodes[4p+2] = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   tnodes[4p+3] = (ptr0, ptr1, 0, 0) as int: internal child index, or ~slot
+// for a leaf (cf. the Compact layout, gpu-ray-traversal/src/rt/cuda/CudaBVH.hpp:41-54).
+// Triangle slots hold (v0, v1-v0, v2-v0); slot i < n is the single-triangle
+// leaf n-1+i, slots n.. hold the emitted lists in list order.  ttri_id[slot]
+// is the original triangle index with bit 31 set on a leaf's last triangle.
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace atr {
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// ------------------------------------------------------------------ scan
+__global__ __launch_bounds__(kScanBlock) void k_scan_tile_sums(const uint32_t* __restrict__ in,
+                                                               uint32_t n,
+                                                               uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t tmp[4];
+    const uint32_t base = blockIdx.x * (uint32_t)kScanTile;
+    uint32_t s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t i = base + (uint32_t)j * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t total;
+    block256_exclusive_sum(s, tmp, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(uint32_t* sums, uint32_t count,
+                                                          uint32_t* total_out)
+{
+    __shared__ uint32_t tmp[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < count; base += kScanBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < count ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block256_exclusive_sum(v, tmp, &total);
+        if (i < count) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+// Tile-local exclusive scan in blocked order (thread t owns items
+// t*16 .. t*16+15 of the tile) plus the tile's prefix.
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* __restrict__ in,
+                                                           uint32_t n,
+                                                           const uint32_t* __restrict__ sums,
+                                                           uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t tmp[4];
+    const uint32_t base = blockIdx.x * (uint32_t)kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t i = base + j;
+        v[j] = i < n ? in[i] : 0u;
+        s += v[j];
+    }
+    uint32_t run = sums[blockIdx.x] + block256_exclusive_sum(s, tmp, nullptr);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ cost
+// copyTreeKernel + calculateCostKernel (TreeCollapser.cu:13-93) in one
+// bottom-up pass.  flags bit0 = collapsible, bit1 = "a climb leaves this
+// node" (leaf, or both children deliver and the node is collapsible).
+__global__ __launch_bounds__(256) void k_collapse_cost(
+    uint32_t N, float ci, float ct, const int32_t* __restrict__ parent,
+    const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+    const int32_t* __restrict__ data, const float4* __restrict__ bbmin,
+    const float4* __restrict__ bbmax, const float* __restrict__ tarea, int32_t* c_parent,
+    int32_t* c_left, int32_t* c_right, int32_t* c_data, int32_t* c_count, float4* c_bbmin,
+    float4* c_bbmax, float* cost, uint32_t* flags, int32_t* height, uint32_t* emit_count,
+    uint32_t* counters, CollapseSummary* summary)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    int node = (int)(N - 1 + i);
+    {
+        const float4 mn = bbmin[node];
+        const float4 mx = bbmax[node];
+        c_parent[node] = parent[node];
+        c_left[node] = left[node];
+        c_right[node] = right[node];
+        c_data[node] = data[node];
+        c_bbmin[node] = mn;
+        c_bbmax[node] = mx;
+        c_count[node] = 1;
+        cost[node] = ct * box_area4(mn, mx);
+        flags[node] = 3u;
+        height[node] = 0;
+    }
+    int cur = parent[node];
+    while (cur >= 0) {
+        if (atomic_add_acq_rel(&counters[cur], 1u) == 0u) return;
+        const int l = left[cur];
+        const int r = right[cur];
+        const int cl = c_count[l];
+        const int cr = c_count[r];
+        const int count = cl + cr;
+        const float4 mn = bbmin[cur];
+        const float4 mx = bbmax[cur];
+        const float ar = box_area4(mn, mx);
+        float cc = ci * ar + cost[l] + cost[r];
+        const float asLeaf = ct * ar * (float)count;
+        const bool collapsible = asLeaf < cc;
+        if (collapsible) cc = asLeaf;
+        const uint32_t fl = flags[l];
+        const uint32_t fr = flags[r];
+        const bool both = (fl & 2u) && (fr & 2u);
+        flags[cur] = (collapsible ? 1u : 0u) | ((both && collapsible) ? 2u : 0u);
+        emit_count[cur] = (both && !collapsible)
+                              ? (uint32_t)((cl > 1 ? cl : 0) + (cr > 1 ? cr : 0))
+                              : 0u;
+        c_count[cur] = count;
+        cost[cur] = cc;
+        height[cur] = 1 + max(height[l], height[r]);
+        const int pp = parent[cur];
+        c_parent[cur] = pp;
+        c_left[cur] = l;
+        c_right[cur] = r;
+        c_data[cur] = -1;
+        c_bbmin[cur] = mn;
+        c_bbmax[cur] = mx;
+        if (pp < 0) {
+            summary->cost_root = cc;
+            summary->area_root = tarea[cur];
+            summary->height = height[cur];
+        }
+        node = cur;
+        cur = pp;
+    }
+}
+
+// Right-first DFS of one child subtree into its list (writeSubtreeLeavesNonrecursive,
+// TreeCollapser.cu:95-137, the pending stack living at the end of the list's
+// own slots).  Every tree leaf holds a distinct triangle, so the reference's
+// duplicate check never removes anything and is not repeated.
+__device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off,
+                          const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                          const int32_t* __restrict__ data, const float4* __restrict__ v,
+                          int32_t* c_data, int32_t* c_count, int32_t* tri_index, float4* ttris,
+                          int32_t* ttri_id)
+{
+    int32_t* buf = tri_index + off;
+    buf[cnt - 1] = left[child];
+    buf[cnt - 2] = right[child];
+    int ss = 2;
+    int pos = 0;
+    while (ss > 0) {
+        const int idx = buf[cnt - ss];
+        --ss;
+        if (idx >= (int)N - 1) {
+            const int tri = data[idx];
+            buf[pos] = tri;
+            if (v) {
+            const uint32_t slot = N + off + (uint32_t)pos;
+            const float4 a = v[3ull * tri];
+            const float4 b = v[3ull * tri + 1];
+            const float4 c = v[3ull * tri + 2];
+            ttris[3ull * slot] = a;
+            ttris[3ull * slot + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+            ttris[3ull * slot + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+            ttri_id[slot] = pos == cnt - 1 ? (int32_t)((uint32_t)tri | 0x80000000u) : tri;
+            }
+            ++pos;
+        } else {
+            const int l = left[idx];
+            const int r = right[idx];
+            buf[cnt - 1 - ss++] = l;
+            buf[cnt - 1 - ss++] = r;
+        }
+    }
+    c_data[child] = (int32_t)off;
+    c_count[child] = pos;
+}
+
+__global__ __launch_bounds__(256) void k_collapse_emit(
+    uint32_t N, const uint32_t* __restrict__ emit_count, const uint32_t* __restrict__ emit_off,
+    const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+    const int32_t* __restrict__ data, const float4* __restrict__ v, int32_t* c_data,
+    int32_t* c_count, int32_t* tri_index, float4* ttris, int32_t* ttri_id)
+{
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p + 1 >= N) return;
+    if (emit_count[p] == 0u) return;
+    uint32_t off = emit_off[p];
+    const int l = left[p];
+    const int r = right[p];
+    const int cl = c_count[l];
+    const int cr = c_count[r];
+    if (cl > 1) {
+        emit_list(N, l, cl, off, left, right, data, v, c_data, c_count, tri_index, ttris,
+                  ttri_id);
+        off += (uint32_t)cl;
+    }
+    if (cr > 1)
+        emit_list(N, r, cr, off, left, right, data, v, c_data, c_count, tri_index, ttris,
+                  ttri_id);
+}
+
+// Single-triangle slots (one per tree leaf, in leaf order).
+__global__ __launch_bounds__(256) void k_leaf_slots(uint32_t N, const int32_t* __restrict__ data,
+                                                    const float4* __restrict__ v, float4* ttris,
+                                                    int32_t* ttri_id)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= N) return;
+    const int tri = data[N - 1 + i];
+    const float4 a = v[3ull * tri];
+    const float4 b = v[3ull * tri + 1];
+    const float4 c = v[3ull * tri + 2];
+    ttris[3ull * i] = a;
+    ttris[3ull * i + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+    ttris[3ull * i + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+    ttri_id[i] = (int32_t)((uint32_t)tri | 0x80000000u);
+}
+
+// One 64-byte traversal record per internal node.  Records of nodes inside
+// collapsed subtrees are written too but never reached.
+__global__ __launch_bounds__(256) void k_layout(uint32_t N, const int32_t* __restrict__ c_left,
+                                                const int32_t* __restrict__ c_right,
+                                                const int32_t* __restrict__ c_data,
+                                                const float4* __restrict__ c_bbmin,
+                                                const float4* __restrict__ c_bbmax,
+                                                float4* tnodes)
+{
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p + 1 >= N) return;
+    const int l = c_left[p];
+    const int r = c_right[p];
+    const int last = (int)N - 1;
+    auto ptr = [&](int ch) -> int {
+        if (ch >= last) return ~(ch - last);
+        const int d = c_data[ch];
+        if (d >= 0) return ~((int)N + d);
+        return ch;
+    };
+    const float4 a0 = c_bbmin[l], a1 = c_bbmax[l];
+    const float4 b0 = c_bbmin[r], b1 = c_bbmax[r];
+    tnodes[4ull * p + 0] = make_float4(a0.x, a1.x, a0.y, a1.y);
+    tnodes[4ull * p + 1] = make_float4(b0.x, b1.x, b0.y, b1.y);
+    tnodes[4ull * p + 2] = make_float4(a0.z, a1.z, b0.z, b1.z);
+    tnodes[4ull * p + 3] = make_float4(__int_as_float(ptr(l)), __int_as_float(ptr(r)), 0.0f, 0.0f);
+}
+
+__global__ void k_fill_u32c(uint32_t* p, uint32_t v, uint64_t count)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += 256ull * gridDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
+atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32_t* out,
+                        uint32_t* total)
+{
+    const uint32_t tiles = std::max(div_up(n, kScanTile), 1u);
+    uint32_t* sums = nullptr;
+    ATR_TRY(dalloc_t(ctx, tiles, &sums));
+    ATR_TRY(launch(ctx, "scan_tile_sums", n * 4.0, [&] {
+        k_scan_tile_sums<<<tiles, kScanBlock, 0, ctx->stream>>>(in, n, sums);
+    }));
+    ATR_TRY(launch(ctx, "scan_sums", tiles * 8.0, [&] {
+        k_scan_sums<<<1, kScanBlock, 0, ctx->stream>>>(sums, tiles, total);
+    }));
+    ATR_TRY(launch(ctx, "scan_apply", n * 8.0, [&] {
+        k_scan_apply<<<tiles, kScanBlock, 0, ctx->stream>>>(in, n, sums, out);
+    }));
+    dfree(ctx, sums);
+    return ATR_OK;
+}
+
+atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct,
+                        const float4* verts)
+{
+    const uint32_t N = t->n;
+    const size_t nodes = 2ull * N - 1;
+    float* cost = nullptr;
+    uint32_t* flags = nullptr;
+    int32_t* height = nullptr;
+    uint32_t* emit_count = nullptr;
+    uint32_t* emit_off = nullptr;
+    uint32_t* counters = nullptr;
+    ATR_TRY(dalloc_t(ctx, nodes, &cost));
+    ATR_TRY(dalloc_t(ctx, nodes, &flags));
+    ATR_TRY(dalloc_t(ctx, nodes, &height));
+    ATR_TRY(dalloc_t(ctx, N, &emit_count));
+    ATR_TRY(dalloc_t(ctx, N, &emit_off));
+    ATR_TRY(dalloc_t(ctx, N, &counters));
+    ATR_TRY(launch(ctx, "collapse_zero", N * 8.0, [&] {
+        k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
+            counters, 0u, N);
+        k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
+            emit_count, 0u, N);
+    }));
+    ATR_TRY(launch(ctx, "collapse_cost", (double)N * 2.0 * 100.0, [&] {
+        k_collapse_cost<<<div_up(N, 256), 256, 0, ctx->stream>>>(
+            N, ci, ct, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
+            c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, cost, flags,
+            height, emit_count, counters, c->d_summary);
+    }));
+    ATR_TRY(run_scan_u32(ctx, emit_count, N - 1, emit_off, &c->d_summary->used));
+    if (verts) {
+        ATR_TRY(launch(ctx, "leaf_slots", N * 68.0, [&] {
+            k_leaf_slots<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->data, verts, c->ttris,
+                                                                   c->ttri_id);
+        }));
+    }
+    ATR_TRY(launch(ctx, "collapse_emit", 0.0, [&] {
+        k_collapse_emit<<<div_up(N, 256), 256, 0, ctx->stream>>>(
+            N, emit_count, emit_off, t->left, t->right, t->data, verts, c->data, c->count,
+            c->tri_index, c->ttris, c->ttri_id);
+    }));
+    if (verts) {
+        ATR_TRY(launch(ctx, "layout", (double)(N - 1) * 64.0 * 2.0, [&] {
+            k_layout<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->left, c->right, c->data,
+                                                              c->bbmin, c->bbmax, c->tnodes);
+        }));
+    }
+    dfree(ctx, cost);
+    dfree(ctx, flags);
+    dfree(ctx, height);
+    dfree(ctx, emit_count);
+    dfree(ctx, emit_off);
+    dfree(ctx, counters);
+    return ATR_OK;
+}
+
+}  // namespace atr

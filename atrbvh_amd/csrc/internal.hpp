@@ -1,0 +1,187 @@
+// Internal declarations shared by the host API and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/atrbvh.h"
+
+namespace atr {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+atr_status fail(atr_status code, const std::string& msg);
+
+#define ATR_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return ::atr::fail(ATR_ERR_DEVICE, std::string(#expr) + ": " +              \
+                                                   hipGetErrorString(_e));               \
+    } while (0)
+
+#define ATR_TRY(expr)                                                                    \
+    do {                                                                                 \
+        atr_status _s = (expr);                                                          \
+        if (_s != ATR_OK) return _s;                                                     \
+    } while (0)
+
+// Per-kernel profile record (events resolved lazily).
+struct ProfileRecord {
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    int launches = 0;
+    double total_ms = 0.0;
+    double bytes = 0.0;
+};
+
+}  // namespace atr
+
+struct atr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // caching allocator: freed blocks are kept per exact size
+    std::multimap<size_t, void*> free_blocks;
+    std::map<void*, size_t> live;
+    // profiling
+    bool profiling = false;
+    std::vector<atr::ProfileRecord> prof;
+    std::map<std::string, int> prof_index;
+    std::vector<hipEvent_t> event_pool;
+    // device status word (bounded spin-wait overflow etc.)
+    uint32_t* d_status = nullptr;
+    // pinned host staging for small readbacks, handed out in fixed slots
+    void* h_pinned = nullptr;
+    std::vector<int> pinned_free;
+};
+
+struct atr_scene {
+    atr_context* ctx = nullptr;
+    uint32_t n = 0;
+    float4* verts = nullptr;  // 3n float4
+    float* d_bbox = nullptr;  // 6 floats: min xyz, max xyz
+    float h_bbox[6] = {0, 0, 0, 0, 0, 0};
+    bool bbox_on_host = false;
+};
+
+struct atr_tree {
+    atr_context* ctx = nullptr;
+    atr_scene* scene = nullptr;  // scene the tree was built from (may be NULL)
+    uint32_t n = 0;
+    int32_t* parent = nullptr;
+    int32_t* left = nullptr;
+    int32_t* right = nullptr;
+    int32_t* data = nullptr;
+    float4* bbmin = nullptr;
+    float4* bbmax = nullptr;
+    float* area = nullptr;
+};
+
+// Summary written by the collapse kernels into device memory.
+struct CollapseSummary {
+    float cost_root;
+    float area_root;
+    int32_t height;
+    uint32_t used;
+};
+
+struct atr_collapsed {
+    atr_context* ctx = nullptr;
+    uint32_t n = 0;
+    int32_t* parent = nullptr;
+    int32_t* left = nullptr;
+    int32_t* right = nullptr;
+    int32_t* data = nullptr;
+    int32_t* count = nullptr;
+    float4* bbmin = nullptr;
+    float4* bbmax = nullptr;
+    int32_t* tri_index = nullptr;  // n
+    // traversal layout
+    float4* tnodes = nullptr;    // 4 float4 per internal node
+    float4* ttris = nullptr;     // 3 float4 per slot (v0, e1, e2), 2n slots
+    int32_t* ttri_id = nullptr;  // 2n
+    bool has_layout = false;
+    CollapseSummary* d_summary = nullptr;
+    CollapseSummary h_summary{};
+    int pinned_slot = -1;
+    bool summary_ready = false;
+    hipEvent_t summary_event = nullptr;
+};
+
+namespace atr {
+
+// ------------------------------------------------------------- allocation
+atr_status dalloc(atr_context* ctx, size_t bytes, void** p);
+void dfree(atr_context* ctx, void* p);
+template <typename T>
+atr_status dalloc_t(atr_context* ctx, size_t count, T** p)
+{
+    return dalloc(ctx, count * sizeof(T), reinterpret_cast<void**>(p));
+}
+
+// ------------------------------------------------------------- launches
+// Brackets a kernel launch with profiling events when enabled and checks the
+// launch status.  `bytes` is the algorithmic byte count of the launch.
+struct LaunchScope {
+    atr_context* ctx;
+    int rec;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    LaunchScope(atr_context* c, const char* name, double bytes);
+    atr_status finish();
+};
+
+template <typename F>
+atr_status launch(atr_context* ctx, const char* name, double bytes, F&& f)
+{
+    LaunchScope s(ctx, name, bytes);
+    f();
+    return s.finish();
+}
+
+// ------------------------------------------------------------- kernels
+// LBVH (lbvh.hip)
+atr_status run_scene_bbox(atr_context* ctx, const float4* verts, uint32_t n, float* d_bbox);
+atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const float* d_bbox,
+                      int bits, void* keys, uint32_t* idx, uint32_t* block_hist,
+                      uint32_t hist_blocks, uint64_t* fill64, uint32_t fill64_count,
+                      uint32_t* zero32, uint32_t zero32_count);
+atr_status run_sort(atr_context* ctx, int key_bits, void* keys, uint32_t* vals, uint32_t n,
+                    void* keys_alt, uint32_t* vals_alt, bool vals_identity,
+                    const uint32_t* block_hist, uint32_t hist_blocks, bool* result_in_alt);
+uint32_t encode_blocks(uint32_t n);
+atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
+                     uint32_t n, const float4* verts, atr_tree* t, uint64_t* slots);
+atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_t* counters);
+
+// Treelets (treelet.hip)
+atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
+                           float ct, int32_t* tris, float* nsah, uint32_t* counters);
+atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
+                          int32_t* tris, float* nsah, uint32_t* counters,
+                          const int32_t* d_schedule, int rounds);
+
+// Collapse + traversal layout (collapse.hip)
+// verts may be NULL: then only the reference collapsed tree is produced
+// (no traversal layout).
+atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct,
+                        const float4* verts);
+atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32_t* out,
+                        uint32_t* total);
+
+// Traversal + ray generation (trace.hip)
+atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uint32_t r,
+                     int any_hit, int32_t* results, int depth);
+atr_status run_rays_pinhole(atr_context* ctx, const float* eye, const float* basis,
+                            float tan_half, uint32_t w, uint32_t h, float tmax, float4* rays);
+atr_status run_rays_diffuse(atr_context* ctx, const float4* verts, const float4* primary,
+                            const int32_t* results, uint32_t r, uint32_t seed, float tmax,
+                            float4* rays);
+
+// Host schedules (schedule.cpp)
+int trbvh_schedule(int treelet, int warp, std::vector<int32_t>& out);
+
+}  // namespace atr

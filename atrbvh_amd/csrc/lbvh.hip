@@ -1,0 +1,625 @@
+// LBVH on gfx950: scene box, Morton encode (+ fused per-block digit
+// histograms), onesweep LSD radix sort of (key, index), and a fused
+// bottom-up radix-tree build + AABB refit that reproduces the Karras node
+// numbering of BVHRT-Core/src/LBVH.cu:208-316 exactly.
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace atr {
+
+namespace {
+
+constexpr int kEncBlock = 256;
+constexpr int kEncItems = 16;
+constexpr int kEncTile = kEncBlock * kEncItems;  // triangles per encode block
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;  // keys per onesweep tile
+constexpr int kSortWaves = kSortBlock / 64;
+
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+
+// ---------------------------------------------------------------- scene box
+// Orderable unsigned image of a float: unsigned order equals float order.
+__device__ __forceinline__ uint32_t f2o(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o)
+{
+    uint32_t b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(b);
+}
+
+__global__ void k_bbox_init(uint32_t* acc)
+{
+    const int t = threadIdx.x;
+    // minimum starts at FLT_MAX, maximum at FLT_MIN (SceneConverter.cpp:28-34)
+    if (t < 3) acc[t] = f2o(FLT_MAX);
+    else if (t < 6) acc[t] = f2o(FLT_MIN);
+}
+
+__global__ __launch_bounds__(256) void k_bbox(const float4* __restrict__ v, uint64_t count,
+                                              uint32_t* acc)
+{
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float mx[3] = {FLT_MIN, FLT_MIN, FLT_MIN};
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += 256ull * gridDim.x) {
+        float4 p = v[i];
+        mn[0] = fminf(mn[0], p.x);
+        mn[1] = fminf(mn[1], p.y);
+        mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x);
+        mx[1] = fmaxf(mx[1], p.y);
+        mx[2] = fmaxf(mx[2], p.z);
+    }
+    __shared__ uint32_t s[6];
+    if (threadIdx.x < 6) s[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+    __syncthreads();
+    for (int a = 0; a < 3; ++a) {
+        atomicMin(&s[a], f2o(mn[a]));
+        atomicMax(&s[3 + a], f2o(mx[a]));
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicMin(&acc[threadIdx.x], s[threadIdx.x]);
+    else if (threadIdx.x < 6) atomicMax(&acc[threadIdx.x], s[threadIdx.x]);
+}
+
+__global__ void k_bbox_final(const uint32_t* acc, float* bbox)
+{
+    if (threadIdx.x < 6) bbox[threadIdx.x] = o2f(acc[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------- encode
+template <typename K>
+struct MortonTraits;
+template <>
+struct MortonTraits<uint32_t> {
+    static constexpr int kPasses = 4;
+    __device__ static uint32_t code(float x, float y, float z)
+    {
+        // calculateMortonCode (Commons.cuh:497-510)
+        uint32_t qx = (uint32_t)fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+        uint32_t qy = (uint32_t)fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+        uint32_t qz = (uint32_t)fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+        return spread10(qx) * 4u + spread10(qy) * 2u + spread10(qz);
+    }
+};
+template <>
+struct MortonTraits<uint64_t> {
+    static constexpr int kPasses = 8;
+    __device__ static uint64_t code(float x, float y, float z)
+    {
+        // calculateMortonCode64 (Commons.cuh:627-640)
+        uint64_t qx = (uint64_t)fminf(fmaxf(x * 2097152.0f, 0.0f), 2097151.0f);
+        uint64_t qy = (uint64_t)fminf(fmaxf(y * 2097152.0f, 0.0f), 2097151.0f);
+        uint64_t qz = (uint64_t)fminf(fmaxf(z * 2097152.0f, 0.0f), 2097151.0f);
+        return spread21(qx) * 4ull + spread21(qy) * 2ull + spread21(qz);
+    }
+};
+
+// One thread per triangle, kEncItems triangles per thread.  Each wave keeps
+// a private LDS histogram of every 8-bit digit of every sort pass; the block
+// total is written to block_hist[block][pass][digit] (no global atomics).
+// The same launch fills scratch the later stages need (slot words to ~0,
+// counters to 0) so no separate memset is enqueued.
+template <typename K>
+__global__ __launch_bounds__(kEncBlock) void k_encode(
+    const float4* __restrict__ v, uint32_t n, const float* __restrict__ bbox, K* __restrict__ keys,
+    uint32_t* __restrict__ idx, uint32_t* __restrict__ block_hist, uint64_t* __restrict__ fill64,
+    uint32_t fill64_count, uint32_t* __restrict__ zero32, uint32_t zero32_count)
+{
+    constexpr int P = MortonTraits<K>::kPasses;
+    __shared__ uint32_t hist[kEncBlock / 64][P * 256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    if (block_hist) {
+        for (int k = tid; k < (kEncBlock / 64) * P * 256; k += kEncBlock) (&hist[0][0])[k] = 0;
+    }
+    __syncthreads();
+    const float bminx = bbox[0], bminy = bbox[1], bminz = bbox[2];
+    const float ex = bbox[3] - bminx, ey = bbox[4] - bminy, ez = bbox[5] - bminz;
+    const uint32_t base = blockIdx.x * (uint32_t)kEncTile;
+#pragma unroll 4
+    for (int j = 0; j < kEncItems; ++j) {
+        const uint32_t i = base + (uint32_t)j * kEncBlock + (uint32_t)tid;
+        if (i < n) {
+            const float4 a = v[3ull * i];
+            const float4 b = v[3ull * i + 1];
+            const float4 c = v[3ull * i + 2];
+            // triangle AABB centroid (LBVH.cu:14-40), normalised (Commons.cuh:426-434)
+            float cx = (fmaxf(fmaxf(a.x, b.x), c.x) + fminf(fminf(a.x, b.x), c.x)) * 0.5f;
+            float cy = (fmaxf(fmaxf(a.y, b.y), c.y) + fminf(fminf(a.y, b.y), c.y)) * 0.5f;
+            float cz = (fmaxf(fmaxf(a.z, b.z), c.z) + fminf(fminf(a.z, b.z), c.z)) * 0.5f;
+            const K key = MortonTraits<K>::code((cx - bminx) / ex, (cy - bminy) / ey,
+                                                (cz - bminz) / ez);
+            keys[i] = key;
+            if (idx) idx[i] = i;
+            if (block_hist) {
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    atomicAdd(&hist[wave][p * 256 + (uint32_t)((key >> (8 * p)) & 255u)], 1u);
+            }
+        }
+    }
+    const uint32_t gsize = gridDim.x * kEncBlock;
+    const uint32_t gtid = blockIdx.x * kEncBlock + tid;
+    for (uint32_t k = gtid; k < fill64_count; k += gsize) fill64[k] = ~0ull;
+    for (uint32_t k = gtid; k < zero32_count; k += gsize) zero32[k] = 0u;
+    if (block_hist) {
+        __syncthreads();
+        uint32_t* out = block_hist + (size_t)blockIdx.x * P * 256;
+        for (int e = tid; e < P * 256; e += kEncBlock) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < kEncBlock / 64; ++w) s += hist[w][e];
+            out[e] = s;
+        }
+    }
+}
+
+// Global digit offsets for every pass: offsets[p][d] = number of keys whose
+// digit p is smaller than d.  One block per pass, one thread per digit.
+__global__ __launch_bounds__(256) void k_digit_offsets(const uint32_t* __restrict__ block_hist,
+                                                       uint32_t blocks, int passes,
+                                                       uint32_t* __restrict__ offsets)
+{
+    __shared__ uint32_t tmp[4];
+    const int p = blockIdx.x;
+    const int d = threadIdx.x;
+    uint32_t s = 0;
+    for (uint32_t b = 0; b < blocks; ++b) s += block_hist[((size_t)b * passes + p) * 256 + d];
+    uint32_t ex = block256_exclusive_sum(s, tmp, nullptr);
+    offsets[p * 256 + d] = ex;
+}
+
+// Histogram of one digit position straight from keys (used when the keys
+// did not come from the encode kernel).
+template <typename K>
+__global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, uint32_t n,
+                                                   int passes, uint32_t* __restrict__ block_hist)
+{
+    __shared__ uint32_t hist[4][8 * 256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    for (int k = tid; k < 4 * 8 * 256; k += 256) (&hist[0][0])[k] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)kEncTile;
+    for (int j = 0; j < kEncItems; ++j) {
+        const uint32_t i = base + (uint32_t)j * 256 + tid;
+        if (i < n) {
+            const K key = keys[i];
+            for (int p = 0; p < passes; ++p)
+                atomicAdd(&hist[wave][p * 256 + (uint32_t)((key >> (8 * p)) & 255u)], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t* out = block_hist + (size_t)blockIdx.x * passes * 256;
+    for (int e = tid; e < passes * 256; e += 256)
+        out[e] = hist[0][e] + hist[1][e] + hist[2][e] + hist[3][e];
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += 256ull * gridDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------- onesweep
+// One stable LSD pass over an 8-bit digit.  Tiles are taken in launch order
+// from an atomic ticket; within a tile each wave ranks its 16 x 64 keys with
+// ballot-based peer masks (keys keep their order: tile, wave, item, lane), a
+// per-digit scan across waves gives tile-local positions, and a decoupled
+// look-back over earlier tiles' published counts gives global positions.
+// Keys are reordered through LDS so writes of equal digits are contiguous.
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_onesweep(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ digit_off,
+    uint32_t* status, uint32_t* tile_ticket, uint32_t* err)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave_hist[kSortWaves][256];
+    __shared__ uint32_t s_local[256];
+    __shared__ uint32_t s_global[256];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ K s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0) s_tile = atomic_add_relaxed(tile_ticket, 1u);
+    for (int k = tid; k < kSortWaves * 256; k += kSortBlock) (&s_wave_hist[0][0])[k] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t tile_base = tile * (uint32_t)kSortTile;
+    const uint32_t wbase = tile_base + (uint32_t)wave * (64u * kSortItems);
+
+    K key[kSortItems];
+    uint32_t val[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t e = wbase + (uint32_t)j * 64u + (uint32_t)lane;
+        if (e < n) {
+            key[j] = kin[e];
+            val[j] = vin ? vin[e] : e;
+        } else {
+            key[j] = 0;
+            val[j] = 0;
+        }
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t e = wbase + (uint32_t)j * 64u + (uint32_t)lane;
+        const bool valid = e < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t r = 0;
+        if (valid) {
+            const uint32_t pre = (uint32_t)__popcll(peers & lt);
+            const uint32_t before = s_wave_hist[wave][d];
+            r = before + pre;
+            if (pre == 0) s_wave_hist[wave][d] = before + (uint32_t)__popcll(peers);
+        }
+        rank[j] = r;
+    }
+    __syncthreads();
+
+    // thread == digit from here on
+    const uint32_t d = (uint32_t)tid;
+    uint32_t count = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t c = s_wave_hist[w][d];
+        s_wave_hist[w][d] = count;
+        count += c;
+    }
+    uint32_t* st = status + (size_t)tile * 256u;
+    store_relaxed(&st[d], (tile == 0 ? kFlagInc : kFlagAgg) | count);
+    const uint32_t local = block256_exclusive_sum(count, s_tmp, nullptr);
+    s_local[d] = local;
+
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int t = (int)tile - 1;
+        uint32_t spins = 0;
+        while (t >= 0) {
+            const uint32_t s = load_relaxed(&status[(size_t)t * 256u + d]);
+            const uint32_t flag = s & ~kValueMask;
+            if (flag == 0u) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += s & kValueMask;
+            if (flag == kFlagInc) break;
+            --t;
+        }
+        store_relaxed(&st[d], kFlagInc | (excl + count));
+    }
+    s_global[d] = digit_off[d] + excl;
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t e = wbase + (uint32_t)j * 64u + (uint32_t)lane;
+        if (e < n) {
+            const uint32_t dj = (uint32_t)(key[j] >> shift) & 255u;
+            const uint32_t pos = s_local[dj] + s_wave_hist[wave][dj] + rank[j];
+            s_keys[pos] = key[j];
+            s_vals[pos] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t valid_count = min((uint32_t)kSortTile, n - tile_base);
+    for (uint32_t k = (uint32_t)tid; k < valid_count; k += kSortBlock) {
+        const K kk = s_keys[k];
+        const uint32_t dd = (uint32_t)(kk >> shift) & 255u;
+        const uint32_t o = s_global[dd] + (k - s_local[dd]);
+        if (o < n) {  // always true unless a bounded look-back gave up (flagged in err)
+            kout[o] = kk;
+            vout[o] = s_vals[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- build
+// delta(i) = common-prefix length of sorted keys i and i+1, with equal keys
+// falling back to the index prefix (longestCommonPrefix, LBVH.cu:163-201).
+__device__ __forceinline__ int delta_at(const uint32_t* k, int i)
+{
+    const uint32_t a = k[i], b = k[i + 1];
+    return a != b ? __clz(a ^ b) : 32 + __clz((uint32_t)(i ^ (i + 1)));
+}
+__device__ __forceinline__ int delta_at(const uint64_t* k, int i)
+{
+    const uint64_t a = k[i], b = k[i + 1];
+    return a != b ? __clzll(a ^ b) : 64 + __clz((uint32_t)(i ^ (i + 1)));
+}
+
+// Fused radix-tree build and AABB refit.  One thread per leaf climbs: the node
+// covering sorted range [lo, hi] has its parent split at hi (it is the left
+// child) when delta(hi) > delta(lo-1), otherwise at lo-1 (right child).  In
+// Karras numbering a left child is stored at its range's right end and a right
+// child at its left end, the root at 0, so every internal node's index is
+// known the moment it is completed.  The two children of a split meet through
+// one 64-bit exchange on slots[split] (first arrival parks {far end, node};
+// the second takes it and builds the parent).
+template <typename K>
+__global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
+                                               const uint32_t* __restrict__ idx, uint32_t n,
+                                               const float4* __restrict__ v, int32_t* parent,
+                                               int32_t* left, int32_t* right, int32_t* data,
+                                               float4* bbmin, float4* bbmax, float* area,
+                                               uint64_t* slots)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const int last = (int)n - 1;
+    const uint32_t tri = idx[i];
+    const float4 a = v[3ull * tri];
+    const float4 b = v[3ull * tri + 1];
+    const float4 c = v[3ull * tri + 2];
+    // calculateLeafBoundingBox (LBVH.cu:430-446)
+    float4 mn = make_float4(fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y),
+                            fminf(fminf(a.z, b.z), c.z), 0.0f);
+    float4 mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
+                            fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
+    int node = last + (int)i;
+    data[node] = (int32_t)tri;
+    left[node] = -1;
+    right[node] = -1;
+    bbmin[node] = mn;
+    bbmax[node] = mx;
+    area[node] = box_area4(mn, mx);
+
+    int lo = (int)i, hi = (int)i;
+    int dl = lo > 0 ? delta_at(keys, lo - 1) : -1;
+    int dr = hi < last ? delta_at(keys, hi) : -1;
+    while (true) {
+        if (lo == 0 && hi == last) {
+            parent[node] = -1;
+            break;
+        }
+        const bool is_left = dr > dl;
+        const int split = is_left ? hi : lo - 1;
+        const uint32_t far_end = (uint32_t)(is_left ? lo : hi);
+        const uint64_t mine = ((uint64_t)far_end << 32) | (uint32_t)node;
+        const uint64_t old = atomic_xchg_acq_rel(&slots[split], mine);
+        if (old == ~0ull) break;  // first arrival: the sibling finishes the parent
+        const int sib = (int)(uint32_t)old;
+        const int sfar = (int)(uint32_t)(old >> 32);
+        const float4 smn = bbmin[sib];
+        const float4 smx = bbmax[sib];
+        mn = make_float4(fminf(mn.x, smn.x), fminf(mn.y, smn.y), fminf(mn.z, smn.z), 0.0f);
+        mx = make_float4(fmaxf(mx.x, smx.x), fmaxf(mx.y, smx.y), fmaxf(mx.z, smx.z), 0.0f);
+        const int lchild = is_left ? node : sib;
+        const int rchild = is_left ? sib : node;
+        if (is_left) {
+            hi = sfar;
+            dr = hi < last ? delta_at(keys, hi) : -1;
+        } else {
+            lo = sfar;
+            dl = lo > 0 ? delta_at(keys, lo - 1) : -1;
+        }
+        int p;
+        if (lo == 0 && hi == last) p = 0;
+        else p = dr > dl ? hi : lo;
+        left[p] = lchild;
+        right[p] = rchild;
+        data[p] = -1;
+        parent[lchild] = p;
+        parent[rchild] = p;
+        bbmin[p] = mn;
+        bbmax[p] = mx;
+        area[p] = box_area4(mn, mx);
+        node = p;
+    }
+}
+
+// Bottom-up refit over a fixed topology (leaf boxes from the vertices, unions
+// upwards), for trees whose vertices moved.
+__global__ __launch_bounds__(256) void k_refit(uint32_t n, const float4* __restrict__ v,
+                                               const int32_t* __restrict__ parent,
+                                               const int32_t* __restrict__ left,
+                                               const int32_t* __restrict__ right,
+                                               const int32_t* __restrict__ data, float4* bbmin,
+                                               float4* bbmax, float* area, uint32_t* counters)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    int node = (int)(n - 1 + i);
+    const uint32_t tri = (uint32_t)data[node];
+    const float4 a = v[3ull * tri];
+    const float4 b = v[3ull * tri + 1];
+    const float4 c = v[3ull * tri + 2];
+    float4 mn = make_float4(fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y),
+                            fminf(fminf(a.z, b.z), c.z), 0.0f);
+    float4 mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
+                            fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
+    bbmin[node] = mn;
+    bbmax[node] = mx;
+    area[node] = box_area4(mn, mx);
+    int cur = parent[node];
+    while (cur >= 0) {
+        if (atomic_add_acq_rel(&counters[cur], 1u) == 0u) return;
+        const int other = left[cur] == node ? right[cur] : left[cur];
+        const float4 smn = bbmin[other];
+        const float4 smx = bbmax[other];
+        mn = make_float4(fminf(mn.x, smn.x), fminf(mn.y, smn.y), fminf(mn.z, smn.z), 0.0f);
+        mx = make_float4(fmaxf(mx.x, smx.x), fmaxf(mx.y, smx.y), fmaxf(mx.z, smx.z), 0.0f);
+        bbmin[cur] = mn;
+        bbmax[cur] = mx;
+        area[cur] = box_area4(mn, mx);
+        node = cur;
+        cur = parent[cur];
+    }
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+uint32_t encode_blocks(uint32_t n) { return div_up(n, kEncTile); }
+
+atr_status run_scene_bbox(atr_context* ctx, const float4* verts, uint32_t n, float* d_bbox)
+{
+    uint32_t* acc = nullptr;
+    ATR_TRY(dalloc_t(ctx, 8, &acc));
+    ATR_TRY(launch(ctx, "bbox_init", 0, [&] { k_bbox_init<<<1, 64, 0, ctx->stream>>>(acc); }));
+    const uint64_t count = 3ull * n;
+    const uint32_t grid = std::min<uint32_t>(div_up(count, 256 * 8), 2048u);
+    ATR_TRY(launch(ctx, "bbox", (double)count * 16.0, [&] {
+        k_bbox<<<std::max(grid, 1u), 256, 0, ctx->stream>>>(verts, count, acc);
+    }));
+    ATR_TRY(launch(ctx, "bbox_final", 0, [&] { k_bbox_final<<<1, 64, 0, ctx->stream>>>(acc, d_bbox); }));
+    dfree(ctx, acc);
+    return ATR_OK;
+}
+
+atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const float* d_bbox,
+                      int bits, void* keys, uint32_t* idx, uint32_t* block_hist,
+                      uint32_t hist_blocks, uint64_t* fill64, uint32_t fill64_count,
+                      uint32_t* zero32, uint32_t zero32_count)
+{
+    const uint32_t blocks = std::max(encode_blocks(n), 1u);
+    if (block_hist && hist_blocks != blocks)
+        return fail(ATR_ERR_INTERNAL, "encode histogram block count mismatch");
+    const int kb = bits == 63 ? 8 : 4;
+    const double bytes = (double)n * (48.0 + kb + (idx ? 4.0 : 0.0));
+    if (bits == 63) {
+        return launch(ctx, "encode63", bytes, [&] {
+            k_encode<uint64_t><<<blocks, kEncBlock, 0, ctx->stream>>>(
+                verts, n, d_bbox, (uint64_t*)keys, idx, block_hist, fill64, fill64_count, zero32,
+                zero32_count);
+        });
+    }
+    return launch(ctx, "encode30", bytes, [&] {
+        k_encode<uint32_t><<<blocks, kEncBlock, 0, ctx->stream>>>(
+            verts, n, d_bbox, (uint32_t*)keys, idx, block_hist, fill64, fill64_count, zero32,
+            zero32_count);
+    });
+}
+
+// Stable LSD sort, 8-bit digits over all key bits.  Results alternate between
+// (keys, vals) and (keys_alt, vals_alt); *result_in_alt says where they end.
+// block_hist may come from the encode kernel; when NULL it is computed here.
+atr_status run_sort(atr_context* ctx, int key_bits, void* keys, uint32_t* vals, uint32_t n,
+                    void* keys_alt, uint32_t* vals_alt, bool vals_identity,
+                    const uint32_t* block_hist, uint32_t hist_blocks, bool* result_in_alt)
+{
+    const int passes = key_bits == 64 ? 8 : 4;
+    const int kbytes = key_bits == 64 ? 8 : 4;
+    const uint32_t tiles = std::max(div_up(n, kSortTile), 1u);
+    uint32_t* offsets = nullptr;
+    uint32_t* status = nullptr;
+    uint32_t* own_hist = nullptr;
+    ATR_TRY(dalloc_t(ctx, (size_t)passes * 256, &offsets));
+    const size_t status_words = (size_t)passes * (tiles * 256u + 64u);
+    ATR_TRY(dalloc_t(ctx, status_words, &status));
+    if (!block_hist) {
+        hist_blocks = std::max(div_up(n, kEncTile), 1u);
+        ATR_TRY(dalloc_t(ctx, (size_t)hist_blocks * passes * 256, &own_hist));
+        if (key_bits == 64) {
+            ATR_TRY(launch(ctx, "histogram64", (double)n * kbytes, [&] {
+                k_histogram<uint64_t><<<hist_blocks, 256, 0, ctx->stream>>>(
+                    (const uint64_t*)keys, n, passes, own_hist);
+            }));
+        } else {
+            ATR_TRY(launch(ctx, "histogram32", (double)n * kbytes, [&] {
+                k_histogram<uint32_t><<<hist_blocks, 256, 0, ctx->stream>>>(
+                    (const uint32_t*)keys, n, passes, own_hist);
+            }));
+        }
+        block_hist = own_hist;
+    }
+    ATR_TRY(launch(ctx, "fill_status", (double)status_words * 4.0, [&] {
+        k_fill_u32<<<std::min<uint32_t>(div_up(status_words, 256), 1024u), 256, 0, ctx->stream>>>(
+            status, 0u, status_words);
+    }));
+    ATR_TRY(launch(ctx, "digit_offsets", (double)hist_blocks * passes * 1024.0, [&] {
+        k_digit_offsets<<<passes, 256, 0, ctx->stream>>>(block_hist, hist_blocks, passes, offsets);
+    }));
+    void* kbuf[2] = {keys, keys_alt};
+    uint32_t* vbuf[2] = {vals, vals_alt};
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+        uint32_t* st = status + (size_t)p * (tiles * 256u + 64u);
+        uint32_t* ticket = st + tiles * 256u;
+        const uint32_t* vin = (p == 0 && vals_identity) ? nullptr : vbuf[cur];
+        const double bytes = (double)n * 2.0 * (kbytes + 4.0) - (vin ? 0.0 : 4.0 * n);
+        atr_status s;
+        if (key_bits == 64) {
+            s = launch(ctx, "onesweep64", bytes, [&] {
+                k_onesweep<uint64_t><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur], n,
+                    8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+            });
+        } else {
+            s = launch(ctx, "onesweep32", bytes, [&] {
+                k_onesweep<uint32_t><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur], n,
+                    8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+            });
+        }
+        ATR_TRY(s);
+        cur = 1 - cur;
+    }
+    const bool in_alt = cur == 1;
+    dfree(ctx, offsets);
+    dfree(ctx, status);
+    if (own_hist) dfree(ctx, own_hist);
+    *result_in_alt = in_alt;
+    return ATR_OK;
+}
+
+atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
+                     uint32_t n, const float4* verts, atr_tree* t, uint64_t* slots)
+{
+    const uint32_t grid = div_up(n, 256);
+    const double bytes = (double)n * 176.0;
+    if (bits == 63) {
+        return launch(ctx, "build_refit63", bytes, [&] {
+            k_build<uint64_t><<<grid, 256, 0, ctx->stream>>>(
+                (const uint64_t*)keys, idx, n, verts, t->parent, t->left, t->right, t->data,
+                t->bbmin, t->bbmax, t->area, slots);
+        });
+    }
+    return launch(ctx, "build_refit30", bytes, [&] {
+        k_build<uint32_t><<<grid, 256, 0, ctx->stream>>>((const uint32_t*)keys, idx, n, verts,
+                                                         t->parent, t->left, t->right, t->data,
+                                                         t->bbmin, t->bbmax, t->area, slots);
+    });
+}
+
+atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_t* counters)
+{
+    const uint32_t n = t->n;
+    ATR_TRY(launch(ctx, "fill_counters", (double)(n - 1) * 4.0, [&] {
+        k_fill_u32<<<std::min<uint32_t>(div_up(n, 256), 1024u), 256, 0, ctx->stream>>>(
+            counters, 0u, (uint64_t)(n - 1));
+    }));
+    return launch(ctx, "refit", (double)n * 176.0, [&] {
+        k_refit<<<div_up(n, 256), 256, 0, ctx->stream>>>(n, verts, t->parent, t->left, t->right,
+                                                         t->data, t->bbmin, t->bbmax, t->area,
+                                                         counters);
+    });
+}
+
+}  // namespace atr

@@ -1,0 +1,341 @@
+// Closest-hit / any-hit traversal of the collapsed BVH on gfx950, with
+// Moller-Trumbore intersection (Intersect::RayTriangle,
+// gpu-ray-traversal/src/rt/Util.cpp:50-94) and the children ordering of
+// BVH::traceRecursive (gpu-ray-traversal/src/rt/bvh/BVH.cpp:133-159): when both
+// children are hit, the one with the smaller entry distance goes first.
+// Ray generation for the primary / diffuse workloads.
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace atr {
+
+namespace {
+
+constexpr float kOoeps = 8.271806125530277e-25f;  // 2^-80
+constexpr float kBoxEps = 1.52587890625e-05f;     // 2^-16
+constexpr int kDone = (int)0x80000000;
+
+struct Ray {
+    float ox, oy, oz, tmin, dx, dy, dz, tmax;
+};
+
+__device__ __forceinline__ float safe_inv(float d)
+{
+    return 1.0f / (fabsf(d) > kOoeps ? d : copysignf(kOoeps, d));
+}
+
+// Slab test with conservative widening (entry shrunk and exit grown by 2^-16
+// relative) so rounding never culls a box holding a triangle hit.
+__device__ __forceinline__ bool slab(float lox, float hix, float loy, float hiy, float loz,
+                                     float hiz, float ox, float oy, float oz, float ix, float iy,
+                                     float iz, float tmin, float tmax, float& tnear)
+{
+    const float x0 = (lox - ox) * ix, x1 = (hix - ox) * ix;
+    const float y0 = (loy - oy) * iy, y1 = (hiy - oy) * iy;
+    const float z0 = (loz - oz) * iz, z1 = (hiz - oz) * iz;
+    float lo = fmaxf(tmin, fminf(x0, x1));
+    lo = fmaxf(lo, fminf(y0, y1));
+    lo = fmaxf(lo, fminf(z0, z1));
+    float hi = fminf(tmax, fmaxf(x0, x1));
+    hi = fminf(hi, fmaxf(y0, y1));
+    hi = fminf(hi, fmaxf(z0, z1));
+    tnear = lo;
+    return lo - fabsf(lo) * kBoxEps <= hi + fabsf(hi) * kBoxEps;
+}
+
+// Moller-Trumbore with precomputed edges e1 = v1 - v0, e2 = v2 - v0 (the same
+// fp32 differences the reference forms per test).  dot(a,b) = (ax*bx + ay*by) + az*bz.
+__device__ __forceinline__ bool intersect(const float4& v0, const float4& e1, const float4& e2,
+                                          float ox, float oy, float oz, float dx, float dy,
+                                          float dz, float tmin, float tmax, float& t, float& u,
+                                          float& v)
+{
+    const float px = dy * e2.z - dz * e2.y;
+    const float py = dz * e2.x - dx * e2.z;
+    const float pz = dx * e2.y - dy * e2.x;
+    const float det = e1.x * px + e1.y * py + e1.z * pz;
+    const float tx = ox - v0.x, ty = oy - v0.y, tz = oz - v0.z;
+    float uu = tx * px + ty * py + tz * pz;
+    const float qx = ty * e1.z - tz * e1.y;
+    const float qy = tz * e1.x - tx * e1.z;
+    const float qz = tx * e1.y - ty * e1.x;
+    float vv = dx * qx + dy * qy + dz * qz;
+    if (det > 0.0f) {
+        if (uu < 0.0f || uu > det) return false;
+        if (vv < 0.0f || uu + vv > det) return false;
+    } else if (det < 0.0f) {
+        if (uu > 0.0f || uu < det) return false;
+        if (vv > 0.0f || uu + vv < det) return false;
+    } else {
+        return false;
+    }
+    const float inv = 1.0f / det;
+    const float tt = (e2.x * qx + e2.y * qy + e2.z * qz) * inv;
+    if (!(tt > tmin && tt < tmax)) return false;
+    t = tt;
+    u = uu * inv;
+    v = vv * inv;
+    return true;
+}
+
+// One thread per ray.  `STACK` entries of private stack; the host only uses
+// this kernel when the tree height fits, otherwise k_trace_deep.
+template <bool ANY, int STACK>
+__global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ rays, uint32_t R,
+                                               const float4* __restrict__ nodes,
+                                               const float4* __restrict__ tris,
+                                               const int32_t* __restrict__ tri_id,
+                                               int4* __restrict__ results, int* __restrict__ gstack,
+                                               int gdepth)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= R) return;
+    const float4 r0 = rays[2ull * i];
+    const float4 r1 = rays[2ull * i + 1];
+    const float ox = r0.x, oy = r0.y, oz = r0.z, tmin = r0.w;
+    const float dx = r1.x, dy = r1.y, dz = r1.z;
+    float t = r1.w;
+    const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+    int id = -1;
+    float bu = 0.0f, bv = 0.0f;
+    int stack[STACK > 0 ? STACK : 1];
+    int* gs = gstack ? gstack + (size_t)i * (size_t)gdepth : nullptr;
+    int sp = 0;
+    int node = t > tmin ? 0 : kDone;
+    while (node != kDone) {
+        while (node >= 0) {
+            const float4 n0 = nodes[4ull * node + 0];
+            const float4 n1 = nodes[4ull * node + 1];
+            const float4 n2 = nodes[4ull * node + 2];
+            const float4 n3 = nodes[4ull * node + 3];
+            float t0, t1;
+            const bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, ox, oy, oz, ix, iy, iz, tmin,
+                                 t, t0);
+            const bool h1 = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, ox, oy, oz, ix, iy, iz, tmin,
+                                 t, t1);
+            const int c0 = __float_as_int(n3.x);
+            const int c1 = __float_as_int(n3.y);
+            if (h0 && h1) {
+                const bool swap = t0 > t1;
+                const int nearc = swap ? c1 : c0;
+                const int farc = swap ? c0 : c1;
+                if (STACK > 0)
+                    stack[sp++] = farc;
+                else
+                    gs[sp++] = farc;
+                node = nearc;
+            } else if (h0) {
+                node = c0;
+            } else if (h1) {
+                node = c1;
+            } else {
+                node = sp > 0 ? (STACK > 0 ? stack[--sp] : gs[--sp]) : kDone;
+            }
+        }
+        if (node == kDone) break;
+        // leaf: slots ~node, ~node+1, ... up to the one tagged last
+        bool stop = false;
+        for (int k = ~node;; ++k) {
+            const int tid = tri_id[k];
+            const float4 v0 = tris[3ull * k];
+            const float4 e1 = tris[3ull * k + 1];
+            const float4 e2 = tris[3ull * k + 2];
+            float tt, uu, vv;
+            if (intersect(v0, e1, e2, ox, oy, oz, dx, dy, dz, tmin, t, tt, uu, vv)) {
+                t = tt;
+                bu = uu;
+                bv = vv;
+                id = tid & 0x7fffffff;
+                if (ANY) {
+                    stop = true;
+                    break;
+                }
+            }
+            if (tid < 0) break;
+        }
+        if (stop) break;
+        node = sp > 0 ? (STACK > 0 ? stack[--sp] : gs[--sp]) : kDone;
+    }
+    results[i] = make_int4(id, __float_as_int(t), __float_as_int(bu), __float_as_int(bv));
+}
+
+// ------------------------------------------------------------------ ray gen
+__global__ __launch_bounds__(256) void k_rays_pinhole(float ex, float ey, float ez, float rx,
+                                                      float ry, float rz, float ux, float uy,
+                                                      float uz, float fx, float fy, float fz,
+                                                      float tan_half, uint32_t w, uint32_t h,
+                                                      float tmax, float4* rays)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    const float sx = (2.0f * ((float)x + 0.5f) / (float)w - 1.0f) * tan_half;
+    const float sy = (2.0f * ((float)y + 0.5f) / (float)h - 1.0f) * tan_half;
+    float dx = sx * rx + sy * ux + fx;
+    float dy = sx * ry + sy * uy + fy;
+    float dz = sx * rz + sy * uz + fz;
+    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    dx /= len;
+    dy /= len;
+    dz /= len;
+    rays[2ull * i] = make_float4(ex, ey, ez, 0.0f);
+    rays[2ull * i + 1] = make_float4(dx, dy, dz, tmax);
+}
+
+// Bob Jenkins' 96-bit mix (public domain lookup2 mix), as used by the
+// reference ray generator.
+__device__ __forceinline__ void jenkins_mix(uint32_t& a, uint32_t& b, uint32_t& c)
+{
+    a -= b; a -= c; a ^= (c >> 13);
+    b -= c; b -= a; b ^= (a << 8);
+    c -= a; c -= b; c ^= (b >> 13);
+    a -= b; a -= c; a ^= (c >> 12);
+    b -= c; b -= a; b ^= (a << 16);
+    c -= a; c -= b; c ^= (b >> 5);
+    a -= b; a -= c; a ^= (c >> 3);
+    b -= c; b -= a; b ^= (a << 10);
+    c -= a; c -= b; c ^= (b >> 15);
+}
+
+// One cosine-weighted ray per primary ray (rayGenAOKernel with numSamples = 1,
+// gpu-ray-traversal/src/rt/ray/RayGenKernels.cu:114-221): origin backed off by
+// 1e-4 along the primary ray, geometric normal flipped towards the ray, frame
+// rotated by a hashed angle, Halton (2,3) sample 1 warped to the hemisphere.
+// A primary miss gives a degenerate ray (tmax = -1).
+__global__ __launch_bounds__(256) void k_rays_diffuse(const float4* __restrict__ v,
+                                                      const float4* __restrict__ prim,
+                                                      const int4* __restrict__ res, uint32_t R,
+                                                      uint32_t seed, float tmax, float4* rays)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= R) return;
+    const float4 r0 = prim[2ull * i];
+    const float4 r1 = prim[2ull * i + 1];
+    const int4 hit = res[i];
+    const float ht = __int_as_float(hit.y);
+    const float back = fmaxf(ht - 1.0e-4f, 0.0f);
+    const float ox = r0.x + r1.x * back, oy = r0.y + r1.y * back, oz = r0.z + r1.z * back;
+    float nx = 1.0f, ny = 0.0f, nz = 0.0f;
+    const int tri = hit.x;
+    if (tri >= 0) {
+        const float4 a = v[3ull * tri], b = v[3ull * tri + 1], c = v[3ull * tri + 2];
+        const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+        const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+        nx = e1y * e2z - e1z * e2y;
+        ny = e1z * e2x - e1x * e2z;
+        nz = e1x * e2y - e1y * e2x;
+        const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+        if (len > 0.0f) {
+            nx /= len;
+            ny /= len;
+            nz /= len;
+        } else {
+            nx = 1.0f;
+            ny = 0.0f;
+            nz = 0.0f;
+        }
+    }
+    if (nx * r1.x + ny * r1.y + nz * r1.z > 0.0f) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    const float ax = fabsf(nx), ay = fabsf(ny), az = fabsf(nz);
+    const float nm = fmaxf(fmaxf(ax, ay), az);
+    float px = ny, py = -nx, pz = 0.0f;
+    if (nm == az) {
+        px = 0.0f;
+        py = nz;
+        pz = -ny;
+    } else if (nm == ax) {
+        px = -nz;
+        py = 0.0f;
+        pz = nx;
+    }
+    const float pl = sqrtf(px * px + py * py + pz * pz);
+    px /= pl;
+    py /= pl;
+    pz /= pl;
+    const float bx = ny * pz - nz * py, by = nz * px - nx * pz, bz = nx * py - ny * px;
+    uint32_t ha = seed + i, hb = 0x9e3779b9u, hc = 0x9e3779b9u;
+    jenkins_mix(ha, hb, hc);
+    jenkins_mix(ha, hb, hc);
+    const float angle = 2.0f * 3.14159265358979f * (float)hc * 2.3283064365386963e-10f;
+    const float ca = cosf(angle), sa = sinf(angle);
+    const float t0x = px * ca + bx * sa, t0y = py * ca + by * sa, t0z = pz * ca + bz * sa;
+    const float t1x = -px * sa + bx * ca, t1y = -py * sa + by * ca, t1z = -pz * sa + bz * ca;
+    // Halton sample index 1: base 2 -> 0.5, base 3 -> 1/3
+    const float hx = 0.5f, hy = 1.0f / 3.0f;
+    const float phi = 2.0f * 3.14159265358979f * hy;
+    const float rr = sqrtf(hx);
+    const float sx = rr * cosf(phi), sy = rr * sinf(phi);
+    const float sz = sqrtf(1.0f - sx * sx - sy * sy);
+    float dx = sx * t0x + sy * t1x + sz * nx;
+    float dy = sx * t0y + sy * t1y + sz * ny;
+    float dz = sx * t0z + sy * t1z + sz * nz;
+    const float dl = sqrtf(dx * dx + dy * dy + dz * dz);
+    dx /= dl;
+    dy /= dl;
+    dz /= dl;
+    rays[2ull * i] = make_float4(ox, oy, oz, 0.0f);
+    rays[2ull * i + 1] = make_float4(dx, dy, dz, tri >= 0 ? tmax : -1.0f);
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uint32_t r,
+                     int any_hit, int32_t* results, int depth)
+{
+    if (r == 0) return ATR_OK;
+    const uint32_t grid = div_up(r, 256);
+    int4* out = (int4*)results;
+    const double bytes = (double)r * 48.0;
+    if (depth < 64) {
+        return launch(ctx, any_hit ? "trace_any" : "trace", bytes, [&] {
+            if (any_hit)
+                k_trace<true, 64><<<grid, 256, 0, ctx->stream>>>(rays, r, c->tnodes, c->ttris,
+                                                                 c->ttri_id, out, nullptr, 0);
+            else
+                k_trace<false, 64><<<grid, 256, 0, ctx->stream>>>(rays, r, c->tnodes, c->ttris,
+                                                                  c->ttri_id, out, nullptr, 0);
+        });
+    }
+    int* gstack = nullptr;
+    const int gdepth = depth + 1;
+    ATR_TRY(dalloc_t(ctx, (size_t)r * gdepth, &gstack));
+    atr_status s = launch(ctx, any_hit ? "trace_any_deep" : "trace_deep", bytes, [&] {
+        if (any_hit)
+            k_trace<true, 0><<<grid, 256, 0, ctx->stream>>>(rays, r, c->tnodes, c->ttris,
+                                                            c->ttri_id, out, gstack, gdepth);
+        else
+            k_trace<false, 0><<<grid, 256, 0, ctx->stream>>>(rays, r, c->tnodes, c->ttris,
+                                                             c->ttri_id, out, gstack, gdepth);
+    });
+    dfree(ctx, gstack);
+    return s;
+}
+
+atr_status run_rays_pinhole(atr_context* ctx, const float* eye, const float* basis,
+                            float tan_half, uint32_t w, uint32_t h, float tmax, float4* rays)
+{
+    const uint64_t count = (uint64_t)w * h;
+    return launch(ctx, "rays_pinhole", count * 32.0, [&] {
+        k_rays_pinhole<<<div_up(count, 256), 256, 0, ctx->stream>>>(
+            eye[0], eye[1], eye[2], basis[0], basis[1], basis[2], basis[3], basis[4], basis[5],
+            basis[6], basis[7], basis[8], tan_half, w, h, tmax, rays);
+    });
+}
+
+atr_status run_rays_diffuse(atr_context* ctx, const float4* verts, const float4* primary,
+                            const int32_t* results, uint32_t r, uint32_t seed, float tmax,
+                            float4* rays)
+{
+    return launch(ctx, "rays_diffuse", r * 96.0, [&] {
+        k_rays_diffuse<<<div_up(r, 256), 256, 0, ctx->stream>>>(
+            verts, primary, (const int4*)results, r, seed, tmax, rays);
+    });
+}
+
+}  // namespace atr

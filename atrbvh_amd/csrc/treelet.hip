@@ -1,0 +1,582 @@
+// Treelet restructuring on gfx950 (wave64): ATRBVH agglomerative clustering
+// (BVHRT-Core/src/AgglomerativeTreelet.cu:392-554) and TRBVH subset DP
+// (BVHRT-Core/src/TRBVH.cu:329-479).
+//
+// Both run the reference's bottom-up pass: one thread per leaf climbs through
+// per-node arrival counters; the second arrival computes the node's triangle
+// count and SAH and, when the count reaches gamma, the node roots a treelet.
+// A wave of 64 lanes takes the voting treelet roots two at a time, one per
+// 32-lane half.  Treelets voted in the same round are disjoint subtrees, so
+// processing two concurrently equals the reference's one-at-a-time loop.
+// Inside a half every cross-lane reduction is a width-32 __shfl_down halving
+// tree with the reference's widths and strict comparisons, so ties resolve to
+// the same lane (findLeafToExpand Treelet.cuh:23-39, findMinimum
+// AgglomerativeTreelet.cu:65-81, reduceOptimal TRBVH.cu:23-35).
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace atr {
+
+namespace {
+
+// (row << 8 | col) of entry p of a lower-triangular matrix stored row by row
+// without its diagonal: p = row*(row-1)/2 + col (TriangularMatrix.cuh:34-52).
+struct TrmTable {
+    uint16_t rc[496];
+};
+constexpr TrmTable make_trm()
+{
+    TrmTable t{};
+    int p = 0;
+    for (int r = 1; r < 32; ++r)
+        for (int c = 0; c < r; ++c) t.rc[p++] = (uint16_t)((r << 8) | c);
+    return t;
+}
+__constant__ TrmTable c_trm = make_trm();
+
+__device__ __forceinline__ int trm(int row, int col) { return ((row - 1) * row) / 2 + col; }
+
+// Per-half LDS layout shared by both kernels' treelet formation.
+struct HalfTreelet {
+    int32_t* internal;  // [32]
+    int32_t* leaf;      // [32]
+    float* area;        // [32]
+};
+
+// formTreelet (Treelet.cuh:57-114) on one 32-lane half.  Lane 0 of the half
+// performs the expansions; the largest stored area among the current treelet
+// leaves (real leaves count as -FLT_MAX) is found with the reference's
+// non-power-of-two halving tree.
+__device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int root, int n,
+                                             uint32_t N, const int32_t* left,
+                                             const int32_t* right, const float* area)
+{
+    if (hl == 0) {
+        const int l = left[root];
+        const int r = right[root];
+        h.internal[0] = root;
+        h.leaf[0] = l;
+        h.leaf[1] = r;
+        h.area[0] = area[l];
+        h.area[1] = area[r];
+    }
+    for (int it = 0; it < n - 2; ++it) {
+        const int m = it + 2;
+        int idx = hl;
+        float a = -INFINITY;
+        if (hl < m) {
+            a = h.area[hl];
+            if (h.leaf[hl] >= (int)N - 1) a = -FLT_MAX;
+        }
+        int cnt = m;
+        int sh = m >> 1;
+        while (cnt > 1) {
+            const int oi = __shfl_down(idx, sh, 32);
+            const float oa = __shfl_down(a, sh, 32);
+            if (oa > a) {
+                a = oa;
+                idx = oi;
+            }
+            cnt = (cnt + 1) >> 1;
+            sh = cnt >> 1;
+        }
+        if (hl == 0) {
+            const int rep = h.leaf[idx];
+            const int l = left[rep];
+            const int r = right[rep];
+            h.internal[it + 1] = rep;
+            h.leaf[idx] = l;
+            h.leaf[it + 2] = r;
+            h.area[idx] = area[l];
+            h.area[it + 2] = area[r];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ ATRBVH
+// Per-half LDS words: internal 32 + leaf 32 + area 32 + boxes 32*6 + matrix.
+constexpr int kAggFixedWords = 32 + 32 + 32 + 192;
+
+__device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, float root_sah, int n,
+                                               uint32_t N, float ci, int32_t* parent,
+                                               int32_t* left, int32_t* right, float4* bbmin,
+                                               float4* bbmax, float* area, float* nsah)
+{
+    HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 32, lds + 64};
+    float* box = lds + 96;
+    float* D = lds + kAggFixedWords;
+    form_treelet(h, hl, root, n, N, left, right, area);
+
+    int node = -1;
+    float sah = 0.0f;
+    if (hl < n) {
+        node = h.leaf[hl];
+        sah = nsah[node];
+        const float4 a = bbmin[node];
+        const float4 b = bbmax[node];
+        box[hl * 6 + 0] = a.x;
+        box[hl * 6 + 1] = a.y;
+        box[hl * 6 + 2] = a.z;
+        box[hl * 6 + 3] = b.x;
+        box[hl * 6 + 4] = b.y;
+        box[hl * 6 + 5] = b.z;
+    }
+    // calculateDistancesMatrix (AgglomerativeTreelet.cu:18-63): every pair i > j
+    const int tsize = n * (n - 1) / 2;
+    for (int p = hl; p < tsize; p += 32) {
+        const int rc = c_trm.rc[p];
+        const float* bi = box + (rc >> 8) * 6;
+        const float* bj = box + (rc & 255) * 6;
+        D[p] = union_area(bi[0], bi[1], bi[2], bi[3], bi[4], bi[5], bj[0], bj[1], bj[2], bj[3],
+                          bj[4], bj[5]);
+    }
+
+    for (int L = n - 1; L > 0; --L) {
+        // findMinimumDistance (:83-99)
+        const int size = L * (L + 1) / 2;
+        float md = FLT_MAX;
+        int mi = 0;
+        for (int j = hl; j < size; j += 32) {
+            const float dd = D[j];
+            if (dd < md) {
+                md = dd;
+                mi = j;
+            }
+        }
+#pragma unroll
+        for (int sh = 16; sh > 0; sh >>= 1) {
+            const float om = __shfl_down(md, sh, 32);
+            const int oi = __shfl_down(mi, sh, 32);
+            if (om < md) {
+                md = om;
+                mi = oi;
+            }
+        }
+        const int minIndex = __shfl(mi, 0, 32);
+        const int rc = c_trm.rc[minIndex];
+        const int r = rc >> 8;
+        const int c = rc & 255;
+
+        // updateState (:101-150)
+        const float s = __shfl(sah, r, 32) + __shfl(sah, c, 32);
+        const int li = __shfl(node, r, 32);
+        const int ri = __shfl(node, c, 32);
+        float amn[3], amx[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            amn[k] = fminf(box[c * 6 + k], box[r * 6 + k]);
+            amx[k] = fmaxf(box[c * 6 + 3 + k], box[r * 6 + 3 + k]);
+        }
+        if (hl == c) {
+            node = h.internal[L - 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                box[c * 6 + k] = amn[k];
+                box[c * 6 + 3 + k] = amx[k];
+            }
+            const float ar = box_area(amn[0], amn[1], amn[2], amx[0], amx[1], amx[2]);
+            const float v = s + ci * ar;
+            h.area[L] = v;
+            sah = v;
+        }
+        const int lastIndex = __shfl(node, L, 32);
+        const float sahLast = __shfl(sah, L, 32);
+        float bmn[3], bmx[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bmn[k] = box[L * 6 + k];
+            bmx[k] = box[L * 6 + 3 + k];
+        }
+        if (hl == r) {
+            node = lastIndex;
+            sah = sahLast;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                box[r * 6 + k] = bmn[k];
+                box[r * 6 + 3 + k] = bmx[k];
+            }
+        }
+        if (hl == L) {
+            node = li;
+            h.leaf[L] = ri;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                box[L * 6 + k] = amn[k];
+                box[L * 6 + 3 + k] = amx[k];
+            }
+        }
+        // updateDistancesMatrix (:152-188)
+        if (L > 1) {
+            const bool cpy = hl < L && hl != r;
+            float cp = 0.0f;
+            if (cpy) cp = D[trm(L, hl)];
+            if (cpy) D[trm(max(r, hl), min(r, hl))] = cp;
+            if (hl < L && hl != c) {
+                const int dr = max(c, hl), dc = min(c, hl);
+                const float* bi = box + dr * 6;
+                const float* bj = box + dc * 6;
+                D[trm(dr, dc)] = union_area(bi[0], bi[1], bi[2], bi[3], bi[4], bi[5], bj[0],
+                                            bj[1], bj[2], bj[3], bj[4], bj[5]);
+            }
+        }
+    }
+
+    // updateTreelet (:190-212)
+    if (h.area[1] < root_sah) {
+        if (hl >= 1 && hl < n) {
+            const int ni = h.internal[hl - 1];
+            const int rr = h.leaf[hl];
+            left[ni] = node;
+            right[ni] = rr;
+            parent[node] = ni;
+            parent[rr] = ni;
+            nsah[ni] = h.area[hl];
+            const float4 mn = make_float4(box[hl * 6 + 0], box[hl * 6 + 1], box[hl * 6 + 2], 0.0f);
+            const float4 mx = make_float4(box[hl * 6 + 3], box[hl * 6 + 4], box[hl * 6 + 5], 0.0f);
+            bbmin[ni] = mn;
+            bbmax[ni] = mx;
+            area[ni] = box_area4(mn, mx);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ TRBVH
+constexpr int kTrN = 7;
+constexpr int kTrSub = 1 << kTrN;
+// internal 8 + leaf 8 + area 8 + boxes 48 + subset boxes 768 + costs 128 +
+// masks 128 + stack 16
+constexpr int kTrWords = 8 + 8 + 8 + 48 + kTrSub * 6 + kTrSub + kTrSub + 16;
+
+__device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, float root_sah,
+                                              int root_tris, uint32_t N, float ci, float ct,
+                                              const int32_t* sched, int rounds, int32_t* parent,
+                                              int32_t* left, int32_t* right, float4* bbmin,
+                                              float4* bbmax, float* area, float* nsah)
+{
+    HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 8, lds + 16};
+    float* box = lds + 24;
+    float* sbox = box + 48;
+    float* cost = sbox + kTrSub * 6;
+    int32_t* mask = (int32_t*)(cost + kTrSub);
+    int32_t* stk = mask + kTrSub;
+    // formTreelet writes at most index n-1 = 6 into the 8-entry arrays
+    form_treelet(h, hl, root, kTrN, N, left, right, area);
+
+    if (hl < kTrN) {
+        const int node = h.leaf[hl];
+        const float4 a = bbmin[node];
+        const float4 b = bbmax[node];
+        box[hl * 6 + 0] = a.x;
+        box[hl * 6 + 1] = a.y;
+        box[hl * 6 + 2] = a.z;
+        box[hl * 6 + 3] = b.x;
+        box[hl * 6 + 4] = b.y;
+        box[hl * 6 + 5] = b.z;
+    }
+    // calculateSubsetSurfaceAreas (TRBVH.cu:37-98): lane covers 4 subsets
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int s = hl * 4 + j;
+        float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+        float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (int i = 0; i < kTrN; ++i) {
+            if (s & (1 << i)) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    mn[k] = fminf(box[i * 6 + k], mn[k]);
+                    mx[k] = fmaxf(box[i * 6 + 3 + k], mx[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sbox[s * 6 + k] = mn[k];
+            sbox[s * 6 + 3 + k] = mx[k];
+        }
+        cost[s] = box_area(mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]);
+        mask[s] = 0;
+    }
+    // leaf costs (:438-443)
+    if (hl < kTrN) cost[1 << hl] = nsah[h.leaf[hl]];
+    const float T = (float)root_tris;
+
+    // processSchedule (:100-134): subsets of size 2..n-2, one per lane per round
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int S = sched[rd * 32 + hl];
+        if (S != 0) {
+            float best = FLT_MAX;
+            int bp = 0;
+            const int delta = (S - 1) & S;
+            int p = (-delta) & S;
+            while (p != 0) {
+                const float cc = cost[p] + cost[p ^ S];
+                if (cc < best) {
+                    best = cc;
+                    bp = p;
+                }
+                p = (p - delta) & S;
+            }
+            const float a = cost[S];
+            cost[S] = fminf(ci * a + best, ct * a * T);
+            mask[S] = bp;
+        }
+    }
+
+    // processSubsets, size n-1 (:139-187): 4 lanes per subset
+    {
+        const int k = hl >> 2;
+        const int full = kTrSub - 1;
+        const int S = full & ~(1 << (k < kTrN ? k : 0));
+        const int leftMask = -(1 << max(k - 1, 0));
+        float best = FLT_MAX;
+        int bp = 0;
+        if (hl < 4 * kTrN) {
+            int base = (hl & 3) + 1;
+            int p = (((base & leftMask) << 1) | (base & ~leftMask)) << 1;
+            const int parts = (1 << (kTrN - 2)) - 1;
+            for (int j = hl & 3; j < parts; j += 4) {
+                const float cc = cost[p] + cost[p ^ S];
+                if (cc < best) {
+                    best = cc;
+                    bp = p;
+                }
+                base += 4;
+                p = (((base & leftMask) << 1) | (base & ~leftMask)) << 1;
+            }
+        }
+#pragma unroll
+        for (int sh = 2; sh > 0; sh >>= 1) {
+            const float oc = __shfl_down(best, sh, 32);
+            const int op = __shfl_down(bp, sh, 32);
+            if (oc < best) {
+                best = oc;
+                bp = op;
+            }
+        }
+        if (hl < 4 * kTrN && (hl & 3) == 0) {
+            const float a = cost[S];
+            cost[S] = fminf(ci * a + best, ct * a * T);
+            mask[S] = bp;
+        }
+    }
+    // size n (:192-219): 32 lanes
+    {
+        const int full = kTrSub - 1;
+        float best = FLT_MAX;
+        int bp = 0;
+        int p = (hl + 1) * 2;
+        const int parts = (1 << (kTrN - 1)) - 1;
+        for (int j = hl; j < parts; j += 32) {
+            const float cc = cost[p] + cost[p ^ full];
+            if (cc < best) {
+                best = cc;
+                bp = p;
+            }
+            p += 64;
+        }
+#pragma unroll
+        for (int sh = 16; sh > 0; sh >>= 1) {
+            const float oc = __shfl_down(best, sh, 32);
+            const int op = __shfl_down(bp, sh, 32);
+            if (oc < best) {
+                best = oc;
+                bp = op;
+            }
+        }
+        if (hl == 0) {
+            const float a = cost[full];
+            cost[full] = fminf(ci * a + best, ct * a * T);
+            mask[full] = bp;
+        }
+    }
+
+    // updateTreelet (:222-327).  Each stack round assigns treelet internal
+    // nodes first to the internal left children in lane order, then to the
+    // internal right children in lane order; new stack entries follow the
+    // same order.  Ballot + prefix popcount make this deterministic.
+    const int full = kTrSub - 1;
+    if (cost[full] < root_sah) {
+        const int halfShift = (__lane_id() >> 5) * 32;
+        const uint32_t ltm = (1u << hl) - 1u;
+        if (hl == 0) {
+            stk[0] = h.internal[0];
+            stk[8] = full;
+        }
+        int sp = 1;
+        int next = 1;
+        while (sp > 0) {
+            const bool act = hl < sp;
+            int sroot = 0, S = 0, P = 0, Q = 0;
+            if (act) {
+                sroot = stk[hl];
+                S = stk[8 + hl];
+                P = mask[S];
+                Q = P ^ S;
+            }
+            const bool lint = act && __popc((uint32_t)P) > 1;
+            const bool rint = act && __popc((uint32_t)Q) > 1;
+            const uint32_t bl = (uint32_t)(__ballot(lint) >> halfShift);
+            const uint32_t br = (uint32_t)(__ballot(rint) >> halfShift);
+            const int nl = __popc(bl);
+            const int nr = __popc(br);
+            if (act) {
+                int child;
+                if (lint) {
+                    const int slot = __popc(bl & ltm);
+                    child = h.internal[next + slot];
+                    const float4 mn = make_float4(sbox[P * 6 + 0], sbox[P * 6 + 1], sbox[P * 6 + 2], 0.0f);
+                    const float4 mx = make_float4(sbox[P * 6 + 3], sbox[P * 6 + 4], sbox[P * 6 + 5], 0.0f);
+                    bbmin[child] = mn;
+                    bbmax[child] = mx;
+                    area[child] = box_area4(mn, mx);
+                    nsah[child] = cost[P];
+                    stk[slot] = child;
+                    stk[8 + slot] = P;
+                } else {
+                    child = h.leaf[__ffs(P) - 1];
+                }
+                left[sroot] = child;
+                parent[child] = sroot;
+                if (rint) {
+                    const int slot = nl + __popc(br & ltm);
+                    child = h.internal[next + slot];
+                    const float4 mn = make_float4(sbox[Q * 6 + 0], sbox[Q * 6 + 1], sbox[Q * 6 + 2], 0.0f);
+                    const float4 mx = make_float4(sbox[Q * 6 + 3], sbox[Q * 6 + 4], sbox[Q * 6 + 5], 0.0f);
+                    bbmin[child] = mn;
+                    bbmax[child] = mx;
+                    area[child] = box_area4(mn, mx);
+                    // the reference stores costs[partition] for the right child too (TRBVH.cu:309)
+                    nsah[child] = cost[P];
+                    stk[slot] = child;
+                    stk[8 + slot] = Q;
+                } else {
+                    child = h.leaf[__ffs(Q) - 1];
+                }
+                right[sroot] = child;
+                parent[child] = sroot;
+            }
+            next += nl + nr;
+            sp = nl + nr;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ driver
+template <bool AGG>
+__global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gamma, float ci,
+                                                      float ct, int32_t* parent, int32_t* left,
+                                                      int32_t* right, float4* bbmin,
+                                                      float4* bbmax, float* area, int32_t* tris,
+                                                      float* nsah, uint32_t* counters,
+                                                      const int32_t* sched, int rounds,
+                                                      int half_words)
+{
+    extern __shared__ float smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int half = lane >> 5;
+    const int hl = lane & 31;
+    float* lds = smem + ((tid >> 6) * 2 + half) * half_words;
+
+    const uint32_t gid = blockIdx.x * 256u + (uint32_t)tid;
+    int cur = -1;
+    if (gid < N) {
+        const int leaf = (int)(N - 1 + gid);
+        tris[leaf] = 1;
+        nsah[leaf] = ct * area[leaf];
+        cur = parent[leaf];
+    }
+    while (__ballot(cur >= 0) != 0ull) {
+        int tc = 0;
+        float my_sah = 0.0f;
+        if (cur >= 0) {
+            if (atomic_add_acq_rel(&counters[cur], 1u) == 0u) {
+                cur = -1;
+            } else {
+                const int l = left[cur];
+                const int r = right[cur];
+                const float ar = area[cur];
+                const int tl = tris[l];
+                const float sl = nsah[l];
+                const int tr = tris[r];
+                const float sr = nsah[r];
+                tc = tl + tr;
+                tris[cur] = tc;
+                my_sah = ci * ar + sl + sr;
+                nsah[cur] = my_sah;
+            }
+        }
+        uint64_t vote = __ballot(tc >= gamma);
+        while (vote != 0ull) {
+            const int r0 = __ffsll((unsigned long long)vote) - 1;
+            vote &= vote - 1ull;
+            int r1 = -1;
+            if (vote != 0ull) {
+                r1 = __ffsll((unsigned long long)vote) - 1;
+                vote &= vote - 1ull;
+            }
+            const int my_r = half ? r1 : r0;
+            const int src = my_r >= 0 ? my_r : r0;
+            const int root = __shfl(cur, src, 64);
+            const float root_sah = __shfl(my_sah, src, 64);
+            const int root_tris = __shfl(tc, src, 64);
+            if (my_r >= 0) {
+                if (AGG)
+                    atrbvh_treelet(lds, hl, root, root_sah, n, N, ci, parent, left, right, bbmin,
+                                   bbmax, area, nsah);
+                else
+                    trbvh_treelet(lds, hl, root, root_sah, root_tris, N, ci, ct, sched, rounds,
+                                  parent, left, right, bbmin, bbmax, area, nsah);
+            }
+        }
+        if (cur >= 0) cur = parent[cur];
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint64_t count)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += 256ull * gridDim.x)
+        p[i] = 0u;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+atr_status zero_counters(atr_context* ctx, uint32_t* counters, uint32_t count)
+{
+    return launch(ctx, "zero_counters", count * 4.0, [&] {
+        k_zero_u32<<<std::min<uint32_t>(std::max(div_up(count, 256), 1u), 2048u), 256, 0,
+                     ctx->stream>>>(counters, count);
+    });
+}
+
+}  // namespace
+
+atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
+                           float ct, int32_t* tris, float* nsah, uint32_t* counters)
+{
+    const uint32_t N = t->n;
+    ATR_TRY(zero_counters(ctx, counters, N - 1));
+    const int half_words = kAggFixedWords + treelet * (treelet - 1) / 2;
+    const size_t lds = (size_t)half_words * 8 * sizeof(float);
+    return launch(ctx, "atrbvh_pass", 0.0, [&] {
+        k_treelet_pass<true><<<div_up(N, 256), 256, lds, ctx->stream>>>(
+            N, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax, t->area,
+            tris, nsah, counters, nullptr, 0, half_words);
+    });
+}
+
+atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
+                          int32_t* tris, float* nsah, uint32_t* counters,
+                          const int32_t* d_schedule, int rounds)
+{
+    const uint32_t N = t->n;
+    ATR_TRY(zero_counters(ctx, counters, N - 1));
+    const size_t lds = (size_t)kTrWords * 8 * sizeof(float);
+    return launch(ctx, "trbvh_pass", 0.0, [&] {
+        k_treelet_pass<false><<<div_up(N, 256), 256, lds, ctx->stream>>>(
+            N, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax, t->area,
+            tris, nsah, counters, d_schedule, rounds, kTrWords);
+    });
+}
+
+}  // namespace atr

@@ -1,0 +1,220 @@
+/*
+ * atrbvh.h -- C ABI of the MI355X (gfx950) BVH builder and traverser.
+ *
+ * Drop-in for the BVHRT-Core build path of leonardo-domingues/atrbvh.  Every
+ * entry point names the reference interface it replaces (paths relative to
+ * the reference repository root).  Conventions:
+ *   - every function returns atr_status; atr_last_error() gives a
+ *     thread-local message for the last failure on the calling thread;
+ *   - all GPU work is ordered on the context's HIP stream and is
+ *     asynchronous, except *_download, atr_sync and the calls documented as
+ *     blocking;
+ *   - handles are created and destroyed by the library; a context is used by
+ *     one host thread at a time;
+ *   - trees use the BVHRT SoA layout: internal nodes 0..n-2, leaves
+ *     n-1..2n-2, root 0, parent(root) = -1, leaves have left = right = -1,
+ *     internal nodes have data = -1, boxes are float4 with w = 0
+ *     (BVHRT-Core/src/SoABVHTree.h:24-33).
+ */
+#ifndef ATRBVH_H
+#define ATRBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum atr_status {
+    ATR_OK = 0,
+    ATR_ERR_INVALID_ARGUMENT = 1,
+    ATR_ERR_OUT_OF_MEMORY = 2,
+    ATR_ERR_DEVICE = 3,
+    ATR_ERR_UNSUPPORTED = 4,
+    ATR_ERR_INTERNAL = 5
+} atr_status;
+
+typedef struct atr_context atr_context;
+typedef struct atr_scene atr_scene;
+typedef struct atr_tree atr_tree;
+typedef struct atr_collapsed atr_collapsed;
+
+/* Host-side view of a tree (all arrays length 2n-1, boxes float4). */
+typedef struct atr_tree_view {
+    uint32_t num_triangles;
+    int32_t root;
+    int32_t* parent;
+    int32_t* left;
+    int32_t* right;
+    int32_t* data;
+    float* bbmin;
+    float* bbmax;
+    float* area;
+} atr_tree_view;
+
+/* Collapsed tree (BVHRT-Core/src/BVHCollapsedTree.h:9-44). Node arrays are
+ * 2n-1 long, triangle_indices is n long and meaningful up to `used`. */
+typedef struct atr_collapsed_view {
+    uint32_t num_triangles;
+    int32_t root;
+    uint32_t used;
+    int32_t* parent;
+    int32_t* left;
+    int32_t* right;
+    int32_t* data;
+    int32_t* triangle_count;
+    float* bbmin;
+    float* bbmax;
+    int32_t* triangle_indices;
+} atr_collapsed_view;
+
+/* ------------------------------------------------------------------ misc */
+const char* atr_last_error(void);
+const char* atr_version(void);
+
+/* Context: one HIP device, one stream, a caching device allocator.
+ * Replaces the reference's implicit legacy stream and per-call cudaMalloc
+ * (BVHRT-Core/src/TimeKernelExecution.h:14-32, LBVHBuilder.cpp:50-103). */
+atr_status atr_context_create(int device, atr_context** out);
+atr_status atr_context_destroy(atr_context* ctx);
+atr_status atr_sync(atr_context* ctx);
+/* Opaque HIP stream handle (hipStream_t) used by the context. */
+void* atr_context_stream(atr_context* ctx);
+
+/* Device memory helpers for callers without an allocator of their own.
+ * Buffers come from the context's caching allocator. */
+atr_status atr_malloc(atr_context* ctx, size_t bytes, void** dptr);
+atr_status atr_free(atr_context* ctx, void* dptr);
+atr_status atr_memcpy_h2d(atr_context* ctx, void* dst, const void* src, size_t bytes);
+atr_status atr_memcpy_d2h(atr_context* ctx, void* dst, const void* src, size_t bytes);
+atr_status atr_memset_d32(atr_context* ctx, void* dst, uint32_t value, size_t count);
+
+/* Per-kernel timing with HIP events recorded on the context stream.  When
+ * enabled every kernel launch is bracketed by two events; atr_profile_get
+ * resolves them (blocking). */
+atr_status atr_profile_enable(atr_context* ctx, int on);
+atr_status atr_profile_reset(atr_context* ctx);
+int atr_profile_count(atr_context* ctx);
+atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* launches,
+                           double* total_ms, double* bytes);
+
+/* ----------------------------------------------------------------- scene */
+/* Scene (BVHRT-Core/src/Scene.h:12-18) built like SceneWrapper
+ * (BVHRT-Core/src/SceneWrapper.cpp:8-30): vertices are float4[3n] (xyz used).
+ * The vertex array is COPIED to the device (SceneWrapper takes ownership of a
+ * host array instead).  bbmin/bbmax NULL => the scene box is computed on the
+ * device with SceneConverter's rule, including its running maximum starting
+ * at FLT_MIN (gpu-ray-traversal/src/rt/SceneConverter.cpp:28-64). */
+atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t num_triangles,
+                            const float* bbmin, const float* bbmax, atr_scene** out);
+/* Same, from vertices already resident on the device (copied device-to-device). */
+atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices,
+                                   uint32_t num_triangles, const float* bbmin,
+                                   const float* bbmax, atr_scene** out);
+/* Scene box (blocking when it was computed on the device). */
+atr_status atr_scene_bounds(atr_scene* scene, float bbmin[3], float bbmax[3]);
+/* Device pointer of the scene's float4 vertex array (for refit/adjoint). */
+atr_status atr_scene_vertices(atr_scene* scene, float** d_vertices);
+atr_status atr_scene_destroy(atr_scene* scene);
+
+/* --------------------------------------------------------------- builders */
+/* LBVHBuilder(use64BitMortonCodes).BuildTree (BVHRT-Core/src/LBVHBuilder.h:31,
+ * LBVHBuilder.cpp:21-130): Morton encode, stable key+index radix sort, radix
+ * tree, bottom-up AABB refit.  morton_bits: 30 (lbvh) or 63 (lbvh64). */
+atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out);
+
+/* TRBVHOptimizer(treeletSize, iterations).Optimize
+ * (BVHRT-Core/src/TRBVHOptimizer.h:27, TRBVHOptimizer.cpp:21-128).
+ * treelet_size must be 7. */
+atr_status atr_trbvh_optimize(atr_tree* tree, int treelet_size, int iterations, float ci,
+                              float ct);
+
+/* AgglomerativeTreeletOptimizer(treeletSize, iterations).Optimize
+ * (BVHRT-Core/src/AgglomerativeTreeletOptimizer.h:24,
+ *  AgglomerativeTreeletOptimizer.cpp:24-110).  3 <= treelet_size <= 32. */
+atr_status atr_atrbvh_optimize(atr_tree* tree, int treelet_size, int iterations, float ci,
+                               float ct);
+
+/* BVHTreeCollapser::Collapse (BVHRT-Core/src/BVHTreeCollapser.h:39,
+ * BVHTreeCollapser.cpp:21-93).  *sah (nullable) = cost[root]/area[root]; when
+ * sah is non-NULL the call blocks until it is known (as the reference does),
+ * otherwise it is asynchronous and atr_collapsed_stats gives the SAH later.  When the tree is bound to a scene the collapsed tree also carries
+ * the traversal layout atr_trace needs. */
+atr_status atr_collapse(atr_tree* tree, float ci, float ct, atr_collapsed** out, float* sah);
+
+/* SoABVHTree::SAH (BVHRT-Core/src/SoABVHTree.cpp:50-74), same summation order;
+ * blocking (downloads the area array). */
+atr_status atr_tree_sah(atr_tree* tree, float ci, float ct, float* out);
+
+/* BVHTreeInstanceManager::DeviceToHostTree / FreeDeviceTree
+ * (BVHRT-Core/src/BVHTreeInstanceManager.cpp:68-131).  The caller provides
+ * host arrays in `host`; download is blocking. */
+atr_status atr_tree_num_triangles(const atr_tree* tree, uint32_t* n);
+atr_status atr_tree_download(atr_tree* tree, atr_tree_view* host);
+atr_status atr_tree_device_view(atr_tree* tree, atr_tree_view* dev);
+atr_status atr_tree_upload(atr_tree* tree, const atr_tree_view* host);
+atr_status atr_tree_destroy(atr_tree* tree);
+/* Tree from a host view (HostToDeviceTree, BVHTreeInstanceManager.cpp:56-66). */
+atr_status atr_tree_create_from_host(atr_context* ctx, const atr_tree_view* host,
+                                     atr_tree** out);
+/* Trees remember the scene they were built from; collapse uses its vertices
+ * to emit the traversal layout.  Uploaded trees are bound here.  The scene
+ * must outlive the trees bound to it. */
+atr_status atr_tree_bind_scene(atr_tree* tree, atr_scene* scene);
+
+/* DeviceToHostCollapsedTree / FreeDeviceCollapsedTree
+ * (BVHRT-Core/src/BVHTreeInstanceManager.cpp:196-270). Blocking. */
+atr_status atr_collapsed_download(atr_collapsed* c, atr_collapsed_view* host);
+/* SAH (cost[root]/area[root], BVHTreeCollapser.cpp:69-74), tree height and the
+ * number of triangles placed in leaf lists.  Blocking. */
+atr_status atr_collapsed_stats(atr_collapsed* c, float* sah, int* height, uint32_t* used);
+atr_status atr_collapsed_destroy(atr_collapsed* c);
+
+/* -------------------------------------------------------------- traversal */
+/* Closest-hit (any_hit = 0) or any-hit traversal of a collapsed tree with
+ * Moller-Trumbore intersection.  Replaces the `trace` kernel ABI
+ * (gpu-ray-traversal/src/rt/kernels/CudaTracerKernels.hpp:73-86) launched by
+ * CudaTracer::traceBatch (gpu-ray-traversal/src/rt/cuda/CudaTracer.cpp:80-158).
+ * d_rays: device float4[2r] {ox,oy,oz,tmin, dx,dy,dz,tmax}
+ *   (gpu-ray-traversal/src/rt/Util.hpp:62-71);
+ * d_results: device int32[4r] {triangle id or -1, t bits, u bits, v bits};
+ *   a miss stores t = tmax (RayResult, Util.hpp:77-87). */
+atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, int any_hit,
+                     int32_t* d_results);
+
+/* Pinhole primary rays, row-major over a w x h grid: origin `eye`, direction
+ * normalize(((2(x+.5)/w-1)*tan_half, (2(y+.5)/h-1)*tan_half, 1)) rotated into
+ * the frame (right, up, forward) given by `basis` (9 floats, rows). */
+atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float basis[9],
+                            float tan_half_fov, uint32_t width, uint32_t height, float tmax,
+                            float* d_rays);
+/* One cosine-weighted diffuse ray per primary ray, from its hit
+ * (rayGenAOKernel semantics, gpu-ray-traversal/src/rt/ray/RayGenKernels.cu:114-221,
+ * numSamples = 1): misses produce degenerate rays (tmax = -1). */
+atr_status atr_rays_diffuse(atr_context* ctx, atr_scene* scene, const float* d_primary,
+                            const int32_t* d_primary_results, uint32_t num_rays,
+                            uint32_t seed, float tmax, float* d_rays);
+
+/* ------------------------------------------------------- low-level stages */
+/* DeviceGenerateMortonCodes (BVHRT-Core/src/LBVH.h:20-50): keys (u32 for 30,
+ * u64 for 63 bits) and indices (identity; may be NULL) on caller buffers. */
+atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t n,
+                             const float bbmin[3], const float bbmax[3], int morton_bits,
+                             void* d_keys, uint32_t* d_indices);
+/* DeviceSort (BVHRT-Core/src/CubWrapper.h:18-34): stable ascending sort of
+ * (key, value) pairs in place; key_bits 32 or 64 (key width in bits, all bits
+ * sorted).  d_values NULL => values are treated as the identity and not
+ * written. */
+atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t* d_values,
+                          uint32_t n);
+/* Bottom-up refit of every box and area of a tree over the scene vertices
+ * (calculateNodeBoundingBoxesKernel semantics, BVHRT-Core/src/LBVH.cu:465-570)
+ * with the topology left unchanged. */
+atr_status atr_refit(atr_tree* tree, atr_scene* scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATRBVH_H */

@@ -1,0 +1,40 @@
+/*
+ * atrbvh_scenes.h -- seeded synthetic scene and schedule utilities that ship
+ * with the library.  They have no counterpart in the reference (whose scenes
+ * are OBJ files loaded by its Windows framework); they produce the benchmark
+ * and test workloads.
+ *
+ * Random numbers: SplitMix64 seeding a PCG32 (XSH-RR) stream; uniform floats
+ * are (u32 >> 8) * 2^-24; normal deviates use Box-Muller in double and are
+ * rounded to fp32 once.  Vertex arrays are float4[3n] with w = 0.
+ */
+#ifndef ATRBVH_SCENES_H
+#define ATRBVH_SCENES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Triangle soup: centroids uniform in [0,1)^3, each vertex = centroid +
+ * N(0, sigma) per axis. */
+void atr_gen_soup(uint64_t seed, uint32_t n, float sigma, float* vertices);
+
+/* Gaussian-mixture soup: `clusters` centres uniform in [0,1)^3 with per-cluster
+ * sigma log-uniform in [sigma_lo, sigma_hi]; each triangle picks a cluster
+ * uniformly, its centroid ~ N(centre, sigma^2 I), and is an equilateral
+ * triangle of side L ~ LogNormal(ln edge_median, edge_sigma) under a uniform
+ * random rotation (normalised Gaussian quaternion). */
+void atr_gen_gmm(uint64_t seed, uint32_t n, uint32_t clusters, float sigma_lo, float sigma_hi,
+                 float edge_median, float edge_sigma, float* vertices);
+
+/* TRBVH lane schedule (the library's own generator): writes rounds*warp ints
+ * into out when cap allows and returns the number of rounds, or -1. */
+int atr_trbvh_schedule(int treelet, int warp, int32_t* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,203 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle on the same
+seeded inputs (bit-exact for keys, permutations, topology, boxes and areas;
+hit ids exact except ties with t within 1e-6 relative)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import atrbvh_amd as ab
+import oracle
+from helpers import (C1_N, C1_SEED, CAMERA_BASIS, CAMERA_EYE, assert_tree_equal, bits,
+                     check_tree_invariants, collapsed_leaf_sets, compare_hits, tree_arrays)
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene_pair(ctx, v):
+    s = ctx.scene(v)
+    mn, mx = s.bounds()
+    omn, omx = oracle.scene_bbox(v)
+    assert np.array_equal(bits(mn), bits(omn)) and np.array_equal(bits(mx), bits(omx))
+    return s, mn, mx
+
+
+# ------------------------------------------------------------------ encode
+@pytest.mark.parametrize("bits_", [30, 63])
+@pytest.mark.parametrize("n", [1, 3, 4095, 4096, 4097, 70001])
+def test_morton_encode(ctx, bits_, n):
+    v = ab.gen_soup(7 + n, n)
+    mn, mx = oracle.scene_bbox(v)
+    dv = ctx.to_device(v)
+    kb = 8 if bits_ == 63 else 4
+    dk = ctx.buffer(n * kb)
+    di = ctx.buffer(n * 4)
+    ctx.morton_encode(dv, n, mn, mx, bits_, dk, di)
+    keys = dk.download(np.uint64 if bits_ == 63 else np.uint32, n)
+    idx = di.download(np.uint32, n)
+    rk, ri = oracle.encode(v, mn, mx, bits_)
+    assert np.array_equal(keys, rk)
+    assert np.array_equal(idx, ri)
+
+
+# ------------------------------------------------------------------ sort
+@pytest.mark.parametrize("key_bits", [32, 64])
+@pytest.mark.parametrize("n", [2, 17, 4096, 4097, 100003, 1 << 20])
+def test_sort_pairs_stable(ctx, key_bits, n):
+    rng = np.random.default_rng(n + key_bits)
+    dt = np.uint64 if key_bits == 64 else np.uint32
+    hi = (1 << 63) if key_bits == 64 else (1 << 30)
+    keys = rng.integers(0, hi, n, dtype=np.uint64).astype(dt)
+    keys[rng.integers(0, n, n // 3)] = keys[0]  # many duplicates
+    vals = rng.permutation(n).astype(np.uint32)
+    dk = ctx.to_device(keys)
+    dvv = ctx.to_device(vals)
+    ctx.sort_pairs(key_bits, dk, dvv, n)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(dk.download(dt, n), keys[order])
+    assert np.array_equal(dvv.download(np.uint32, n), vals[order])
+
+
+def test_sort_identity_values(ctx):
+    n = 50000
+    keys = np.random.default_rng(5).integers(0, 1000, n).astype(np.uint32)
+    dk = ctx.to_device(keys)
+    ctx.sort_pairs(32, dk, None, n)
+    assert np.array_equal(dk.download(np.uint32, n), np.sort(keys, kind="stable"))
+
+
+# ------------------------------------------------------------------ LBVH
+@pytest.mark.parametrize("bits_", [30, 63])
+@pytest.mark.parametrize("n", [2, 3, 5, 64, 1000, 4097, C1_N])
+def test_lbvh_bit_exact(ctx, bits_, n):
+    v = ab.gen_soup(C1_SEED if n == C1_N else 100 + n, n)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(bits_ == 63).BuildTree(s)
+    dev = t.download()
+    ref = oracle.lbvh(v, mn, mx, bits_)
+    assert_tree_equal(dev, tree_arrays(ref), f"lbvh{bits_}")
+    check_tree_invariants(dev, n)
+    assert t.SAH() == pytest.approx(oracle.sah(ref), rel=0, abs=0)
+    t.close()
+    s.close()
+
+
+def test_lbvh_duplicate_keys(ctx):
+    """Identical triangles (equal Morton codes) split by index."""
+    base = ab.gen_soup(3, 40)
+    v = np.concatenate([base] * 50 + [ab.gen_soup(4, 37)])
+    n = v.shape[0] // 3
+    for bits_ in (30, 63):
+        s, mn, mx = _scene_pair(ctx, v)
+        t = ab.LBVHBuilder(bits_ == 63).BuildTree(s)
+        ref = oracle.lbvh(v, mn, mx, bits_)
+        assert_tree_equal(t.download(), tree_arrays(ref), "dup")
+        t.close()
+        s.close()
+
+
+# ------------------------------------------------------------------ treelets
+@pytest.mark.parametrize("treelet,iters", [(9, 2), (7, 3), (3, 1), (16, 1), (32, 1)])
+def test_atrbvh_bit_exact(ctx, treelet, iters):
+    n = C1_N if (treelet, iters) == (9, 2) else 20000
+    v = ab.gen_soup(C1_SEED if n == C1_N else 55, n)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(False).BuildTree(s)
+    ab.AgglomerativeTreeletOptimizer(treelet, iters).Optimize(t)
+    dev = t.download()
+    ref = oracle.lbvh(v, mn, mx, 30)
+    oracle.atrbvh(ref, treelet, iters)
+    assert_tree_equal(dev, tree_arrays(ref), f"atrbvh{treelet}_{iters}")
+    check_tree_invariants(dev, n)
+    t.close()
+    s.close()
+
+
+@pytest.mark.parametrize("bits_,n", [(30, C1_N), (63, 30000), (30, 9), (30, 300)])
+def test_trbvh_bit_exact(ctx, bits_, n):
+    v = ab.gen_soup(C1_SEED if n == C1_N else 77 + n, n)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(bits_ == 63).BuildTree(s)
+    ab.TRBVHOptimizer(7, 3).Optimize(t)
+    dev = t.download()
+    ref = oracle.lbvh(v, mn, mx, bits_)
+    oracle.trbvh(ref, 7, 3)
+    assert_tree_equal(dev, tree_arrays(ref), "trbvh")
+    check_tree_invariants(dev, n)
+    t.close()
+    s.close()
+
+
+# ------------------------------------------------------------------ collapse + trace
+def _c1_pipeline(ctx):
+    v = ab.gen_soup(C1_SEED, C1_N)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(False).BuildTree(s)
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+    c, sah = ab.BVHTreeCollapser().Collapse(t)
+    ref = oracle.lbvh(v, mn, mx, 30)
+    oracle.atrbvh(ref, 9, 2)
+    rc, rsah = oracle.collapse(ref)
+    return v, s, t, c, sah, ref, rc, rsah
+
+
+def test_collapse_bit_exact(ctx):
+    v, s, t, c, sah, ref, rc, rsah = _c1_pipeline(ctx)
+    dev = c.download()
+    assert bits(np.float32(sah)) == bits(np.float32(rsah))
+    for k in ("parent", "left", "right", "data", "count"):
+        assert np.array_equal(dev[k], getattr(rc, k)), k
+    assert dev["used"] == rc.used
+    assert np.array_equal(dev["tri_index"][: rc.used], rc.tri_index[: rc.used])
+    for k in ("bbmin", "bbmax"):
+        assert np.array_equal(bits(dev[k][:, :3]), bits(getattr(rc, k)[:, :3]))
+    sets = collapsed_leaf_sets(dev, C1_N)
+    tris = sorted(x for s_ in sets.values() for x in s_)
+    assert tris == list(range(C1_N))
+    c.close()
+    t.close()
+    s.close()
+
+
+def test_trace_primary_c1(ctx):
+    v, s, t, c, sah, ref, rc, rsah = _c1_pipeline(ctx)
+    g = 256
+    r = g * g
+    d_rays = ctx.buffer(r * 32)
+    d_res = ctx.buffer(r * 16)
+    ctx.rays_pinhole(CAMERA_EYE, CAMERA_BASIS, 1.0, g, g, np.inf, d_rays)
+    c.trace(d_rays, r, d_res)
+    rays = d_rays.download(np.float32, (r, 8))
+    dev = d_res.download(np.int32, (r, 4))
+    rr = oracle.trace(rc, v, rays)
+    compare_hits(dev, rr)
+    # brute force on a subset pins the oracle traversal itself
+    sub = slice(0, r, 97)
+    compare_hits(rr[sub], oracle.trace_brute(v, rays[sub]))
+    # diffuse secondary rays from the primary hits
+    d_sec = ctx.buffer(r * 32)
+    ctx.rays_diffuse(s, d_rays, d_res, r, 12345, np.inf, d_sec)
+    d_res2 = ctx.buffer(r * 16)
+    c.trace(d_sec, r, d_res2)
+    sec = d_sec.download(np.float32, (r, 8))
+    compare_hits(d_res2.download(np.int32, (r, 4)), oracle.trace(rc, v, sec))
+    c.close()
+    t.close()
+    s.close()
+
+
+def test_trace_any_hit(ctx):
+    v, s, t, c, sah, ref, rc, rsah = _c1_pipeline(ctx)
+    g = 128
+    r = g * g
+    d_rays = ctx.buffer(r * 32)
+    d_res = ctx.buffer(r * 16)
+    ctx.rays_pinhole(CAMERA_EYE, CAMERA_BASIS, 1.0, g, g, np.inf, d_rays)
+    c.trace(d_rays, r, d_res, any_hit=True)
+    dev = d_res.download(np.int32, (r, 4))
+    rays = d_rays.download(np.float32, (r, 8))
+    closest = oracle.trace(rc, v, rays)
+    assert np.array_equal(dev[:, 0] >= 0, closest[:, 0] >= 0)
+    c.close()
+    t.close()
+    s.close()
