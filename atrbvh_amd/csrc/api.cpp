@@ -394,6 +394,7 @@ static atr_status tree_alloc(atr_context* ctx, uint32_t n, atr_tree** out)
     if (s == ATR_OK) s = dalloc_t(ctx, m, &t->bbmin);
     if (s == ATR_OK) s = dalloc_t(ctx, m, &t->bbmax);
     if (s == ATR_OK) s = dalloc_t(ctx, m, &t->area);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &t->span);
     if (s != ATR_OK) {
         atr_tree_destroy(t);
         return s;
@@ -413,6 +414,7 @@ atr_status atr_tree_destroy(atr_tree* t)
     dfree(ctx, t->bbmin);
     dfree(ctx, t->bbmax);
     dfree(ctx, t->area);
+    dfree(ctx, t->span);
     delete t;
     return ATR_OK;
 }
@@ -578,6 +580,20 @@ atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (h->num_triangles != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "size mismatch");
     const size_t m = 2ull * t->n - 1;
+    // topology check: every node but the root is somebody's child exactly once
+    {
+        std::vector<uint8_t> seen(m, 0);
+        for (uint32_t p = 0; p + 1 < t->n; ++p) {
+            for (int32_t ch : {h->left[p], h->right[p]}) {
+                if (ch <= 0 || (size_t)ch >= m || seen[ch])
+                    return fail(ATR_ERR_INVALID_ARGUMENT, "tree view is not a binary tree");
+                seen[ch] = 1;
+            }
+        }
+        for (size_t i = t->n - 1; i < m; ++i)
+            if (h->data[i] < 0 || (uint32_t)h->data[i] >= t->n)
+                return fail(ATR_ERR_INVALID_ARGUMENT, "leaf data index out of range");
+    }
     ATR_HIP(hipStreamSynchronize(t->ctx->stream));
     ATR_HIP(hipMemcpy(t->parent, h->parent, m * 4, hipMemcpyHostToDevice));
     ATR_HIP(hipMemcpy(t->left, h->left, m * 4, hipMemcpyHostToDevice));
@@ -586,6 +602,9 @@ atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
     ATR_HIP(hipMemcpy(t->bbmin, h->bbmin, m * 16, hipMemcpyHostToDevice));
     ATR_HIP(hipMemcpy(t->bbmax, h->bbmax, m * 16, hipMemcpyHostToDevice));
     ATR_HIP(hipMemcpy(t->area, h->area, m * 4, hipMemcpyHostToDevice));
+    std::vector<int2> sp(m);
+    host_spans(t->n, h->left, h->right, sp.data());
+    ATR_HIP(hipMemcpy(t->span, sp.data(), m * sizeof(int2), hipMemcpyHostToDevice));
     return ATR_OK;
 }
 
@@ -679,10 +698,10 @@ atr_status atr_collapsed_destroy(atr_collapsed* c)
     dfree(ctx, c->ttris);
     dfree(ctx, c->ttri_id);
     dfree(ctx, c->d_summary);
-    if (c->summary_event) {
-        (void)hipEventSynchronize(c->summary_event);
-        ctx->event_pool.push_back(c->summary_event);
-    }
+    // The event and pinned slot go back to their pools without waiting: a
+    // later user records / copies after this tree's pending work in stream
+    // order and waits on its own event.
+    if (c->summary_event) ctx->event_pool.push_back(c->summary_event);
     if (c->pinned_slot >= 0) ctx->pinned_free.push_back(c->pinned_slot);
     delete c;
     return ATR_OK;

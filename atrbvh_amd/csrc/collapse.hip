@@ -101,8 +101,12 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     const float4* __restrict__ bbmax, const float* __restrict__ tarea, int32_t* c_parent,
     int32_t* c_left, int32_t* c_right, int32_t* c_data, int32_t* c_count, float4* c_bbmin,
     float4* c_bbmax, float* cost, uint32_t* flags, int32_t* height, uint32_t* emit_count,
-    uint32_t* counters, CollapseSummary* summary)
+    uint32_t* counters, const int2* __restrict__ span, CollapseSummary* summary)
 {
+    __shared__ uint32_t s_cnt[256];
+    s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int blk = (int)blockIdx.x;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
     int node = (int)(N - 1 + i);
@@ -115,36 +119,59 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         c_data[node] = data[node];
         c_bbmin[node] = mn;
         c_bbmax[node] = mx;
-        c_count[node] = 1;
-        cost[node] = ct * box_area4(mn, mx);
-        flags[node] = 3u;
-        height[node] = 0;
+        st_coh(&c_count[node], 1);
+        st_coh(&cost[node], ct * box_area4(mn, mx));
+        st_coh(&flags[node], 3u);
+        st_coh(&height[node], 0);
     }
     int cur = parent[node];
     while (cur >= 0) {
-        if (atomic_add_acq_rel(&counters[cur], 1u) == 0u) return;
+        // block-local nodes meet in LDS (workgroup scope), others globally
+        const bool local = block_local(span[cur], cur, blk);
+        const uint32_t old =
+            local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
+        if (old == 0u) return;
         const int l = left[cur];
         const int r = right[cur];
-        const int cl = c_count[l];
-        const int cr = c_count[r];
+        int cl, cr, hl_, hr_;
+        float kl, kr;
+        uint32_t fl, fr;
+        if (local) {
+            cl = ld<false>(&c_count[l]);
+            cr = ld<false>(&c_count[r]);
+            kl = ld<false>(&cost[l]);
+            kr = ld<false>(&cost[r]);
+            fl = ld<false>(&flags[l]);
+            fr = ld<false>(&flags[r]);
+            hl_ = ld<false>(&height[l]);
+            hr_ = ld<false>(&height[r]);
+        } else {
+            cl = ld<true>(&c_count[l]);
+            cr = ld<true>(&c_count[r]);
+            kl = ld<true>(&cost[l]);
+            kr = ld<true>(&cost[r]);
+            fl = ld<true>(&flags[l]);
+            fr = ld<true>(&flags[r]);
+            hl_ = ld<true>(&height[l]);
+            hr_ = ld<true>(&height[r]);
+        }
         const int count = cl + cr;
         const float4 mn = bbmin[cur];
         const float4 mx = bbmax[cur];
         const float ar = box_area4(mn, mx);
-        float cc = ci * ar + cost[l] + cost[r];
+        float cc = ci * ar + kl + kr;
         const float asLeaf = ct * ar * (float)count;
         const bool collapsible = asLeaf < cc;
         if (collapsible) cc = asLeaf;
-        const uint32_t fl = flags[l];
-        const uint32_t fr = flags[r];
         const bool both = (fl & 2u) && (fr & 2u);
-        flags[cur] = (collapsible ? 1u : 0u) | ((both && collapsible) ? 2u : 0u);
+        st_coh(&flags[cur], (collapsible ? 1u : 0u) | ((both && collapsible) ? 2u : 0u));
         emit_count[cur] = (both && !collapsible)
                               ? (uint32_t)((cl > 1 ? cl : 0) + (cr > 1 ? cr : 0))
                               : 0u;
-        c_count[cur] = count;
-        cost[cur] = cc;
-        height[cur] = 1 + max(height[l], height[r]);
+        st_coh(&c_count[cur], count);
+        st_coh(&cost[cur], cc);
+        const int h = 1 + max(hl_, hr_);
+        st_coh(&height[cur], h);
         const int pp = parent[cur];
         c_parent[cur] = pp;
         c_left[cur] = l;
@@ -155,7 +182,7 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         if (pp < 0) {
             summary->cost_root = cc;
             summary->area_root = tarea[cur];
-            summary->height = height[cur];
+            summary->height = h;
         }
         node = cur;
         cur = pp;
@@ -328,7 +355,7 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
         k_collapse_cost<<<div_up(N, 256), 256, 0, ctx->stream>>>(
             N, ci, ct, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
             c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, cost, flags,
-            height, emit_count, counters, c->d_summary);
+            height, emit_count, counters, t->span, c->d_summary);
     }));
     ATR_TRY(run_scan_u32(ctx, emit_count, N - 1, emit_off, &c->d_summary->used));
     if (verts) {

@@ -89,6 +89,138 @@ __device__ __forceinline__ uint32_t atomic_add_relaxed(uint32_t* p, uint32_t v)
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------- cross-XCD hand-over
+// Bottom-up passes hand results from one climbing thread to another.  When
+// both belong to the same workgroup (a "block-local" node) a workgroup-scope
+// LDS atomic orders them.  When the node's subtree spans several workgroups
+// the two threads may sit on different XCDs, whose L2 caches are not kept
+// coherent; an agent-scope acquire/release there would write back and
+// invalidate the whole XCD L2 on every hand-over.  Instead:
+//   - values a thread on another XCD may read later are written with
+//     device-coherent stores (agent-scope relaxed atomic stores, which go
+//     through to the point of coherence shared by all XCDs);
+//   - before every arrival a thread waits until all of its wave's memory
+//     operations have completed (so those stores are visible device-wide);
+//   - the crossing arrival itself is a relaxed agent-scope atomic, and the
+//     second arrival reads the handed-over values with device-coherent loads
+//     issued only after the atomic has returned.
+__device__ __forceinline__ void wait_memory()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ void st_coh(int32_t* p, int32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(float* p, float v)
+{
+    __hip_atomic_store((uint32_t*)p, __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(int2* p, int2 v)
+{
+    const uint64_t w = ((uint64_t)(uint32_t)v.y << 32) | (uint32_t)v.x;
+    __hip_atomic_store((uint64_t*)p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(float4* p, float4 v)
+{
+    uint64_t* q = (uint64_t*)p;
+    const uint64_t a = ((uint64_t)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+    const uint64_t b = ((uint64_t)__float_as_uint(v.w) << 32) | __float_as_uint(v.z);
+    __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Loads: COH selects the device-coherent flavour (crossing readers).
+template <bool COH>
+__device__ __forceinline__ int32_t ld(const int32_t* p)
+{
+    if constexpr (COH)
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+template <bool COH>
+__device__ __forceinline__ uint32_t ld(const uint32_t* p)
+{
+    if constexpr (COH)
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+template <bool COH>
+__device__ __forceinline__ float ld(const float* p)
+{
+    if constexpr (COH)
+        return __uint_as_float(
+            __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    else
+        return *p;
+}
+template <bool COH>
+__device__ __forceinline__ int2 ld(const int2* p)
+{
+    if constexpr (COH) {
+        const uint64_t w =
+            __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_int2((int)(uint32_t)w, (int)(uint32_t)(w >> 32));
+    } else {
+        return *p;
+    }
+}
+template <bool COH>
+__device__ __forceinline__ float4 ld(const float4* p)
+{
+    if constexpr (COH) {
+        const uint64_t* q = (const uint64_t*)p;
+        const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)),
+                           __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+    } else {
+        return *p;
+    }
+}
+
+// Arrival at a node whose two children were produced inside this workgroup.
+__device__ __forceinline__ uint32_t arrive_local(uint32_t* lds_counter)
+{
+    wait_memory();
+    return __hip_atomic_fetch_add(lds_counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Arrival at a node whose children may have been produced on other XCDs.
+__device__ __forceinline__ uint32_t arrive_crossing(uint32_t* counter)
+{
+    wait_memory();
+    const uint32_t old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    wait_memory();
+    return old;
+}
+
+__device__ __forceinline__ uint64_t exchange_crossing(uint64_t* slot, uint64_t v)
+{
+    wait_memory();
+    const uint64_t old = __hip_atomic_exchange(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_memory();
+    return old;
+}
+
+// Block-local test: every leaf under the node (span) and the node index itself
+// lie in this workgroup's 256-leaf block.
+__device__ __forceinline__ bool block_local(int2 span, int node, int blk)
+{
+    return (span.x >> 8) == blk && (span.y >> 8) == blk && (node >> 8) == blk;
+}
+
 // --------------------------------------------------------------- wave ops
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

@@ -79,6 +79,9 @@ struct atr_tree {
     float4* bbmin = nullptr;
     float4* bbmax = nullptr;
     float* area = nullptr;
+    // [min, max] leaf position (leaf node index - (n-1)) under each node; a
+    // node whose span lies in one 256-leaf block is handled by that block only
+    int2* span = nullptr;
 };
 
 // Summary written by the collapse kernels into device memory.
@@ -156,6 +159,8 @@ uint32_t encode_blocks(uint32_t n);
 atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
                      uint32_t n, const float4* verts, atr_tree* t, uint64_t* slots);
 atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_t* counters);
+// Host: spans of a tree given as host arrays (post-order over the topology).
+void host_spans(uint32_t n, const int32_t* left, const int32_t* right, int2* span);
 
 // Treelets (treelet.hip)
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,

@@ -359,19 +359,80 @@ __device__ __forceinline__ int delta_at(const uint64_t* k, int i)
 // Karras numbering a left child is stored at its range's right end and a right
 // child at its left end, the root at 0, so every internal node's index is
 // known the moment it is completed.  The two children of a split meet through
-// one 64-bit exchange on slots[split] (first arrival parks {far end, node};
-// the second takes it and builds the parent).
+// one 64-bit exchange (first arrival parks {far end, node}; the second takes
+// it and builds the parent).
+//
+// A split whose node lies entirely inside this block's 256 leaves only ever
+// sees this block's threads, so its exchange (and the box handed over) lives
+// in LDS with workgroup-scope ordering.  Only splits whose node crosses a
+// block boundary use the global slot with agent-scope acquire/release (which
+// on gfx950 writes back / invalidates the XCD's L2).  A split s is block-local
+// iff the nearest smaller delta on both sides lies inside the block, i.e. the
+// prefix minimum of delta over [base-1, s-1] and the suffix minimum over
+// [s+1, base+255] are both below delta(s).
+template <typename K>
+__device__ __forceinline__ int delta_or_end(const K* k, int i, int last)
+{
+    return (i < 0 || i >= last) ? -1 : delta_at(k, i);
+}
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
                                                const uint32_t* __restrict__ idx, uint32_t n,
                                                const float4* __restrict__ v, int32_t* parent,
                                                int32_t* left, int32_t* right, int32_t* data,
                                                float4* bbmin, float4* bbmax, float* area,
-                                               uint64_t* slots)
+                                               int2* span, uint64_t* slots)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
+    __shared__ int s_d[257];
+    __shared__ int s_wmin[8];
+    __shared__ uint8_t s_local[256];
+    __shared__ uint64_t s_slot[256];
+    __shared__ float s_box[256][2][6];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int last = (int)n - 1;
+    const int base = (int)(blockIdx.x * 256u);
+    // delta at positions base-1 .. base+255 (index t <-> position base-1+t)
+    s_d[tid] = delta_or_end(keys, base - 1 + tid, last);
+    if (tid == 0) s_d[256] = delta_or_end(keys, base + 255, last);
+    s_slot[tid] = ~0ull;
+    __syncthreads();
+    // inclusive prefix minimum of s_d[0..tid] and suffix minimum of s_d[tid+1..256]
+    int pm = s_d[tid];
+    int sm = s_d[tid + 1];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int a = __shfl_up(pm, o, 64);
+        if (lane >= o) pm = min(pm, a);
+        const int b = __shfl_down(sm, o, 64);
+        if (lane + o < 64) sm = min(sm, b);
+    }
+    if (lane == 63) s_wmin[wave] = pm;
+    if (lane == 0) s_wmin[4 + wave] = sm;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) pm = min(pm, s_wmin[w]);
+    for (int w = wave + 1; w < 4; ++w) sm = min(sm, s_wmin[4 + w]);
+    sm = min(sm, s_d[256]);
+    __syncthreads();
+    // split s = base + tid (tid < 255): left part s_d[0..tid] = positions base-1..s-1,
+    // right part positions s+1..base+255 = s_d[tid+2..256]
+    {
+        const int ds = s_d[tid + 1];
+        int right_min = __shfl_down(sm, 1, 64);
+        // lane 63 needs the next wave's suffix: recompute from the wave minima
+        if (lane == 63) {
+            right_min = s_d[256];
+            for (int w = wave + 1; w < 4; ++w) right_min = min(right_min, s_wmin[4 + w]);
+        }
+        s_local[tid] = (tid < 255 && pm < ds && right_min < ds) ? 1 : 0;
+    }
+    __syncthreads();
+
+    const uint32_t i = (uint32_t)base + (uint32_t)tid;
+    if (i >= n) return;
     const uint32_t tri = idx[i];
     const float4 a = v[3ull * tri];
     const float4 b = v[3ull * tri + 1];
@@ -385,9 +446,11 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
     data[node] = (int32_t)tri;
     left[node] = -1;
     right[node] = -1;
-    bbmin[node] = mn;
-    bbmax[node] = mx;
+    // a sibling on another XCD may read this box at a crossing split
+    st_coh(&bbmin[node], mn);
+    st_coh(&bbmax[node], mx);
     area[node] = box_area4(mn, mx);
+    span[node] = make_int2((int)i, (int)i);
 
     int lo = (int)i, hi = (int)i;
     int dl = lo > 0 ? delta_at(keys, lo - 1) : -1;
@@ -401,12 +464,28 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
         const int split = is_left ? hi : lo - 1;
         const uint32_t far_end = (uint32_t)(is_left ? lo : hi);
         const uint64_t mine = ((uint64_t)far_end << 32) | (uint32_t)node;
-        const uint64_t old = atomic_xchg_acq_rel(&slots[split], mine);
-        if (old == ~0ull) break;  // first arrival: the sibling finishes the parent
+        const int k = split - base;
+        uint64_t old;
+        float4 smn, smx;
+        if (k >= 0 && k < 255 && s_local[k]) {
+            float* my = s_box[k][is_left ? 0 : 1];
+            my[0] = mn.x; my[1] = mn.y; my[2] = mn.z;
+            my[3] = mx.x; my[4] = mx.y; my[5] = mx.z;
+            old = __hip_atomic_exchange(&s_slot[k], mine, __ATOMIC_ACQ_REL,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == ~0ull) break;
+            const float* sb = s_box[k][is_left ? 1 : 0];
+            smn = make_float4(sb[0], sb[1], sb[2], 0.0f);
+            smx = make_float4(sb[3], sb[4], sb[5], 0.0f);
+        } else {
+            old = exchange_crossing(&slots[split], mine);
+            if (old == ~0ull) break;  // first arrival: the sibling finishes the parent
+            const int sib = (int)(uint32_t)old;
+            smn = ld<true>(&bbmin[sib]);
+            smx = ld<true>(&bbmax[sib]);
+        }
         const int sib = (int)(uint32_t)old;
         const int sfar = (int)(uint32_t)(old >> 32);
-        const float4 smn = bbmin[sib];
-        const float4 smx = bbmax[sib];
         mn = make_float4(fminf(mn.x, smn.x), fminf(mn.y, smn.y), fminf(mn.z, smn.z), 0.0f);
         mx = make_float4(fmaxf(mx.x, smx.x), fmaxf(mx.y, smx.y), fmaxf(mx.z, smx.z), 0.0f);
         const int lchild = is_left ? node : sib;
@@ -426,9 +505,10 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
         data[p] = -1;
         parent[lchild] = p;
         parent[rchild] = p;
-        bbmin[p] = mn;
-        bbmax[p] = mx;
+        st_coh(&bbmin[p], mn);
+        st_coh(&bbmax[p], mx);
         area[p] = box_area4(mn, mx);
+        span[p] = make_int2(lo, hi);
         node = p;
     }
 }
@@ -440,8 +520,13 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, const float4* __restr
                                                const int32_t* __restrict__ left,
                                                const int32_t* __restrict__ right,
                                                const int32_t* __restrict__ data, float4* bbmin,
-                                               float4* bbmax, float* area, uint32_t* counters)
+                                               float4* bbmax, float* area, uint32_t* counters,
+                                               const int2* __restrict__ span)
 {
+    __shared__ uint32_t s_cnt[256];
+    s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int blk = (int)blockIdx.x;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     int node = (int)(n - 1 + i);
@@ -453,19 +538,22 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, const float4* __restr
                             fminf(fminf(a.z, b.z), c.z), 0.0f);
     float4 mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
                             fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
-    bbmin[node] = mn;
-    bbmax[node] = mx;
+    st_coh(&bbmin[node], mn);
+    st_coh(&bbmax[node], mx);
     area[node] = box_area4(mn, mx);
     int cur = parent[node];
     while (cur >= 0) {
-        if (atomic_add_acq_rel(&counters[cur], 1u) == 0u) return;
+        const bool local = block_local(span[cur], cur, blk);
+        const uint32_t old =
+            local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
+        if (old == 0u) return;
         const int other = left[cur] == node ? right[cur] : left[cur];
-        const float4 smn = bbmin[other];
-        const float4 smx = bbmax[other];
+        const float4 smn = local ? ld<false>(&bbmin[other]) : ld<true>(&bbmin[other]);
+        const float4 smx = local ? ld<false>(&bbmax[other]) : ld<true>(&bbmax[other]);
         mn = make_float4(fminf(mn.x, smn.x), fminf(mn.y, smn.y), fminf(mn.z, smn.z), 0.0f);
         mx = make_float4(fmaxf(mx.x, smx.x), fmaxf(mx.y, smx.y), fmaxf(mx.z, smx.z), 0.0f);
-        bbmin[cur] = mn;
-        bbmax[cur] = mx;
+        st_coh(&bbmin[cur], mn);
+        st_coh(&bbmax[cur], mx);
         area[cur] = box_area4(mn, mx);
         node = cur;
         cur = parent[cur];
@@ -598,13 +686,14 @@ atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_
         return launch(ctx, "build_refit63", bytes, [&] {
             k_build<uint64_t><<<grid, 256, 0, ctx->stream>>>(
                 (const uint64_t*)keys, idx, n, verts, t->parent, t->left, t->right, t->data,
-                t->bbmin, t->bbmax, t->area, slots);
+                t->bbmin, t->bbmax, t->area, t->span, slots);
         });
     }
     return launch(ctx, "build_refit30", bytes, [&] {
         k_build<uint32_t><<<grid, 256, 0, ctx->stream>>>((const uint32_t*)keys, idx, n, verts,
                                                          t->parent, t->left, t->right, t->data,
-                                                         t->bbmin, t->bbmax, t->area, slots);
+                                                         t->bbmin, t->bbmax, t->area, t->span,
+                                                         slots);
     });
 }
 
@@ -618,7 +707,7 @@ atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_
     return launch(ctx, "refit", (double)n * 176.0, [&] {
         k_refit<<<div_up(n, 256), 256, 0, ctx->stream>>>(n, verts, t->parent, t->left, t->right,
                                                          t->data, t->bbmin, t->bbmax, t->area,
-                                                         counters);
+                                                         counters, t->span);
     });
 }
 

@@ -12,6 +12,8 @@
 #include <cstdint>
 #include <vector>
 
+#include <utility>
+
 #include "internal.hpp"
 
 namespace atr {
@@ -48,6 +50,37 @@ int trbvh_schedule(int treelet, int warp, std::vector<int32_t>& out)
         std::copy(rounds[r].begin(), rounds[r].end(), out.begin() + dst * warp);
     }
     return (int)rounds.size();
+}
+
+}  // namespace atr
+
+namespace atr {
+
+void host_spans(uint32_t n, const int32_t* left, const int32_t* right, int2* span)
+{
+    const uint32_t last = n - 1;
+    for (uint32_t i = last; i < 2 * n - 1; ++i) span[i] = make_int2((int)(i - last), (int)(i - last));
+    // iterative post-order from the root
+    std::vector<std::pair<int32_t, int32_t>> st;
+    st.emplace_back(0, 0);
+    while (!st.empty()) {
+        auto& f = st.back();
+        const int32_t node = f.first;
+        if ((uint32_t)node >= last) {
+            st.pop_back();
+            continue;
+        }
+        if (f.second == 0) {
+            f.second = 1;
+            st.emplace_back(right[node], 0);
+            st.emplace_back(left[node], 0);
+            continue;
+        }
+        st.pop_back();
+        const int2 a = span[left[node]];
+        const int2 b = span[right[node]];
+        span[node] = make_int2(std::min(a.x, b.x), std::max(a.y, b.y));
+    }
 }
 
 }  // namespace atr

@@ -47,18 +47,19 @@ struct HalfTreelet {
 // performs the expansions; the largest stored area among the current treelet
 // leaves (real leaves count as -FLT_MAX) is found with the reference's
 // non-power-of-two halving tree.
+template <bool COH>
 __device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int root, int n,
                                              uint32_t N, const int32_t* left,
                                              const int32_t* right, const float* area)
 {
     if (hl == 0) {
-        const int l = left[root];
-        const int r = right[root];
+        const int l = ld<COH>(&left[root]);
+        const int r = ld<COH>(&right[root]);
         h.internal[0] = root;
         h.leaf[0] = l;
         h.leaf[1] = r;
-        h.area[0] = area[l];
-        h.area[1] = area[r];
+        h.area[0] = ld<COH>(&area[l]);
+        h.area[1] = ld<COH>(&area[r]);
     }
     for (int it = 0; it < n - 2; ++it) {
         const int m = it + 2;
@@ -82,44 +83,49 @@ __device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int r
         }
         if (hl == 0) {
             const int rep = h.leaf[idx];
-            const int l = left[rep];
-            const int r = right[rep];
+            const int l = ld<COH>(&left[rep]);
+            const int r = ld<COH>(&right[rep]);
             h.internal[it + 1] = rep;
             h.leaf[idx] = l;
             h.leaf[it + 2] = r;
-            h.area[idx] = area[l];
-            h.area[it + 2] = area[r];
+            h.area[idx] = ld<COH>(&area[l]);
+            h.area[it + 2] = ld<COH>(&area[r]);
         }
     }
 }
 
 // ------------------------------------------------------------------ ATRBVH
-// Per-half LDS words: internal 32 + leaf 32 + area 32 + boxes 32*6 + matrix.
-constexpr int kAggFixedWords = 32 + 32 + 32 + 192;
+// Per-half LDS words: internal 32 + leaf 32 + area 32 + boxes 32*6 + spans
+// 32*2 + matrix.
+constexpr int kAggFixedWords = 32 + 32 + 32 + 192 + 64;
 
+template <bool COH>
 __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, float root_sah, int n,
                                                uint32_t N, float ci, int32_t* parent,
                                                int32_t* left, int32_t* right, float4* bbmin,
-                                               float4* bbmax, float* area, float* nsah)
+                                               float4* bbmax, float* area, float* nsah,
+                                               int2* span)
 {
     HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 32, lds + 64};
     float* box = lds + 96;
+    int2* sp = (int2*)(lds + 288);
     float* D = lds + kAggFixedWords;
-    form_treelet(h, hl, root, n, N, left, right, area);
+    form_treelet<COH>(h, hl, root, n, N, left, right, area);
 
     int node = -1;
     float sah = 0.0f;
     if (hl < n) {
         node = h.leaf[hl];
-        sah = nsah[node];
-        const float4 a = bbmin[node];
-        const float4 b = bbmax[node];
+        sah = ld<COH>(&nsah[node]);
+        const float4 a = ld<COH>(&bbmin[node]);
+        const float4 b = ld<COH>(&bbmax[node]);
         box[hl * 6 + 0] = a.x;
         box[hl * 6 + 1] = a.y;
         box[hl * 6 + 2] = a.z;
         box[hl * 6 + 3] = b.x;
         box[hl * 6 + 4] = b.y;
         box[hl * 6 + 5] = b.z;
+        sp[hl] = ld<COH>(&span[node]);
     }
     // calculateDistancesMatrix (AgglomerativeTreelet.cu:18-63): every pair i > j
     const int tsize = n * (n - 1) / 2;
@@ -167,8 +173,11 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
             amn[k] = fminf(box[c * 6 + k], box[r * 6 + k]);
             amx[k] = fmaxf(box[c * 6 + 3 + k], box[r * 6 + 3 + k]);
         }
+        const int2 spr = sp[r], spc = sp[c];
+        const int2 spm = make_int2(min(spr.x, spc.x), max(spr.y, spc.y));
         if (hl == c) {
             node = h.internal[L - 1];
+            sp[c] = spm;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 box[c * 6 + k] = amn[k];
@@ -187,9 +196,11 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
             bmn[k] = box[L * 6 + k];
             bmx[k] = box[L * 6 + 3 + k];
         }
+        const int2 spL = sp[L];
         if (hl == r) {
             node = lastIndex;
             sah = sahLast;
+            sp[r] = spL;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 box[r * 6 + k] = bmn[k];
@@ -199,6 +210,7 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
         if (hl == L) {
             node = li;
             h.leaf[L] = ri;
+            sp[L] = spm;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 box[L * 6 + k] = amn[k];
@@ -226,16 +238,17 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
         if (hl >= 1 && hl < n) {
             const int ni = h.internal[hl - 1];
             const int rr = h.leaf[hl];
-            left[ni] = node;
-            right[ni] = rr;
-            parent[node] = ni;
-            parent[rr] = ni;
-            nsah[ni] = h.area[hl];
+            st_coh(&left[ni], node);
+            st_coh(&right[ni], rr);
+            st_coh(&parent[node], ni);
+            st_coh(&parent[rr], ni);
+            st_coh(&nsah[ni], h.area[hl]);
             const float4 mn = make_float4(box[hl * 6 + 0], box[hl * 6 + 1], box[hl * 6 + 2], 0.0f);
             const float4 mx = make_float4(box[hl * 6 + 3], box[hl * 6 + 4], box[hl * 6 + 5], 0.0f);
-            bbmin[ni] = mn;
-            bbmax[ni] = mx;
-            area[ni] = box_area4(mn, mx);
+            st_coh(&bbmin[ni], mn);
+            st_coh(&bbmax[ni], mx);
+            st_coh(&area[ni], box_area4(mn, mx));
+            st_coh(&span[ni], sp[hl]);
         }
     }
 }
@@ -243,35 +256,40 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
 // ------------------------------------------------------------------ TRBVH
 constexpr int kTrN = 7;
 constexpr int kTrSub = 1 << kTrN;
-// internal 8 + leaf 8 + area 8 + boxes 48 + subset boxes 768 + costs 128 +
-// masks 128 + stack 16
-constexpr int kTrWords = 8 + 8 + 8 + 48 + kTrSub * 6 + kTrSub + kTrSub + 16;
+// internal 8 + leaf 8 + area 8 + boxes 48 + leaf spans 16 + subset boxes 768 +
+// subset spans 256 + costs 128 + masks 128 + stack 16
+constexpr int kTrWords = 8 + 8 + 8 + 48 + 16 + kTrSub * 6 + kTrSub * 2 + kTrSub + kTrSub + 16;
 
+template <bool COH>
 __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, float root_sah,
                                               int root_tris, uint32_t N, float ci, float ct,
                                               const int32_t* sched, int rounds, int32_t* parent,
                                               int32_t* left, int32_t* right, float4* bbmin,
-                                              float4* bbmax, float* area, float* nsah)
+                                              float4* bbmax, float* area, float* nsah,
+                                              int2* span)
 {
     HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 8, lds + 16};
     float* box = lds + 24;
-    float* sbox = box + 48;
-    float* cost = sbox + kTrSub * 6;
+    int2* lsp = (int2*)(box + 48);
+    float* sbox = box + 48 + 16;
+    int2* ssp = (int2*)(sbox + kTrSub * 6);
+    float* cost = sbox + kTrSub * 6 + kTrSub * 2;
     int32_t* mask = (int32_t*)(cost + kTrSub);
     int32_t* stk = mask + kTrSub;
     // formTreelet writes at most index n-1 = 6 into the 8-entry arrays
-    form_treelet(h, hl, root, kTrN, N, left, right, area);
+    form_treelet<COH>(h, hl, root, kTrN, N, left, right, area);
 
     if (hl < kTrN) {
         const int node = h.leaf[hl];
-        const float4 a = bbmin[node];
-        const float4 b = bbmax[node];
+        const float4 a = ld<COH>(&bbmin[node]);
+        const float4 b = ld<COH>(&bbmax[node]);
         box[hl * 6 + 0] = a.x;
         box[hl * 6 + 1] = a.y;
         box[hl * 6 + 2] = a.z;
         box[hl * 6 + 3] = b.x;
         box[hl * 6 + 4] = b.y;
         box[hl * 6 + 5] = b.z;
+        lsp[hl] = ld<COH>(&span[node]);
     }
     // calculateSubsetSurfaceAreas (TRBVH.cu:37-98): lane covers 4 subsets
 #pragma unroll
@@ -279,6 +297,7 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
         const int s = hl * 4 + j;
         float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
         float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        int2 ss = make_int2(0x7fffffff, -1);
         for (int i = 0; i < kTrN; ++i) {
             if (s & (1 << i)) {
 #pragma unroll
@@ -286,8 +305,11 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
                     mn[k] = fminf(box[i * 6 + k], mn[k]);
                     mx[k] = fmaxf(box[i * 6 + 3 + k], mx[k]);
                 }
+                ss.x = min(ss.x, lsp[i].x);
+                ss.y = max(ss.y, lsp[i].y);
             }
         }
+        ssp[s] = ss;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             sbox[s * 6 + k] = mn[k];
@@ -297,7 +319,7 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
         mask[s] = 0;
     }
     // leaf costs (:438-443)
-    if (hl < kTrN) cost[1 << hl] = nsah[h.leaf[hl]];
+    if (hl < kTrN) cost[1 << hl] = ld<COH>(&nsah[h.leaf[hl]]);
     const float T = (float)root_tris;
 
     // processSchedule (:100-134): subsets of size 2..n-2, one per lane per round
@@ -426,34 +448,36 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
                     child = h.internal[next + slot];
                     const float4 mn = make_float4(sbox[P * 6 + 0], sbox[P * 6 + 1], sbox[P * 6 + 2], 0.0f);
                     const float4 mx = make_float4(sbox[P * 6 + 3], sbox[P * 6 + 4], sbox[P * 6 + 5], 0.0f);
-                    bbmin[child] = mn;
-                    bbmax[child] = mx;
-                    area[child] = box_area4(mn, mx);
-                    nsah[child] = cost[P];
+                    st_coh(&bbmin[child], mn);
+                    st_coh(&bbmax[child], mx);
+                    st_coh(&area[child], box_area4(mn, mx));
+                    st_coh(&nsah[child], cost[P]);
+                    st_coh(&span[child], ssp[P]);
                     stk[slot] = child;
                     stk[8 + slot] = P;
                 } else {
                     child = h.leaf[__ffs(P) - 1];
                 }
-                left[sroot] = child;
-                parent[child] = sroot;
+                st_coh(&left[sroot], child);
+                st_coh(&parent[child], sroot);
                 if (rint) {
                     const int slot = nl + __popc(br & ltm);
                     child = h.internal[next + slot];
                     const float4 mn = make_float4(sbox[Q * 6 + 0], sbox[Q * 6 + 1], sbox[Q * 6 + 2], 0.0f);
                     const float4 mx = make_float4(sbox[Q * 6 + 3], sbox[Q * 6 + 4], sbox[Q * 6 + 5], 0.0f);
-                    bbmin[child] = mn;
-                    bbmax[child] = mx;
-                    area[child] = box_area4(mn, mx);
+                    st_coh(&bbmin[child], mn);
+                    st_coh(&bbmax[child], mx);
+                    st_coh(&area[child], box_area4(mn, mx));
                     // the reference stores costs[partition] for the right child too (TRBVH.cu:309)
-                    nsah[child] = cost[P];
+                    st_coh(&nsah[child], cost[P]);
+                    st_coh(&span[child], ssp[Q]);
                     stk[slot] = child;
                     stk[8 + slot] = Q;
                 } else {
                     child = h.leaf[__ffs(Q) - 1];
                 }
-                right[sroot] = child;
-                parent[child] = sroot;
+                st_coh(&right[sroot], child);
+                st_coh(&parent[child], sroot);
             }
             next += nl + nr;
             sp = nl + nr;
@@ -468,10 +492,11 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gam
                                                       int32_t* right, float4* bbmin,
                                                       float4* bbmax, float* area, int32_t* tris,
                                                       float* nsah, uint32_t* counters,
-                                                      const int32_t* sched, int rounds,
-                                                      int half_words)
+                                                      int2* span, const int32_t* sched,
+                                                      int rounds, int half_words)
 {
     extern __shared__ float smem[];
+    __shared__ uint32_t s_cnt[256];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int half = lane >> 5;
@@ -479,31 +504,51 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gam
     float* lds = smem + ((tid >> 6) * 2 + half) * half_words;
 
     const uint32_t gid = blockIdx.x * 256u + (uint32_t)tid;
+    const int blk = (int)blockIdx.x;
+    s_cnt[tid] = 0u;
+    __syncthreads();
     int cur = -1;
     if (gid < N) {
         const int leaf = (int)(N - 1 + gid);
-        tris[leaf] = 1;
-        nsah[leaf] = ct * area[leaf];
+        st_coh(&tris[leaf], 1);
+        st_coh(&nsah[leaf], ct * area[leaf]);
         cur = parent[leaf];
     }
     while (__ballot(cur >= 0) != 0ull) {
         int tc = 0;
         float my_sah = 0.0f;
+        bool local = true;
         if (cur >= 0) {
-            if (atomic_add_acq_rel(&counters[cur], 1u) == 0u) {
+            // Nodes whose leaves all belong to this block meet in LDS; the rest
+            // meet in global memory (cross-XCD hand-over, see device.hpp).  A
+            // node's own links and area are unchanged since the last launch
+            // until it is reached, so they are read plainly.
+            local = block_local(span[cur], cur, blk);
+            const uint32_t old =
+                local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
+            if (old == 0u) {
                 cur = -1;
             } else {
                 const int l = left[cur];
                 const int r = right[cur];
                 const float ar = area[cur];
-                const int tl = tris[l];
-                const float sl = nsah[l];
-                const int tr = tris[r];
-                const float sr = nsah[r];
+                int tl, tr;
+                float sl, sr;
+                if (local) {
+                    tl = ld<false>(&tris[l]);
+                    sl = ld<false>(&nsah[l]);
+                    tr = ld<false>(&tris[r]);
+                    sr = ld<false>(&nsah[r]);
+                } else {
+                    tl = ld<true>(&tris[l]);
+                    sl = ld<true>(&nsah[l]);
+                    tr = ld<true>(&tris[r]);
+                    sr = ld<true>(&nsah[r]);
+                }
                 tc = tl + tr;
-                tris[cur] = tc;
+                st_coh(&tris[cur], tc);
                 my_sah = ci * ar + sl + sr;
-                nsah[cur] = my_sah;
+                st_coh(&nsah[cur], my_sah);
             }
         }
         uint64_t vote = __ballot(tc >= gamma);
@@ -520,13 +565,25 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gam
             const int root = __shfl(cur, src, 64);
             const float root_sah = __shfl(my_sah, src, 64);
             const int root_tris = __shfl(tc, src, 64);
+            const bool root_local = __shfl((int)local, src, 64) != 0;
             if (my_r >= 0) {
-                if (AGG)
-                    atrbvh_treelet(lds, hl, root, root_sah, n, N, ci, parent, left, right, bbmin,
-                                   bbmax, area, nsah);
-                else
-                    trbvh_treelet(lds, hl, root, root_sah, root_tris, N, ci, ct, sched, rounds,
-                                  parent, left, right, bbmin, bbmax, area, nsah);
+                if (AGG) {
+                    if (root_local)
+                        atrbvh_treelet<false>(lds, hl, root, root_sah, n, N, ci, parent, left,
+                                              right, bbmin, bbmax, area, nsah, span);
+                    else
+                        atrbvh_treelet<true>(lds, hl, root, root_sah, n, N, ci, parent, left,
+                                             right, bbmin, bbmax, area, nsah, span);
+                } else {
+                    if (root_local)
+                        trbvh_treelet<false>(lds, hl, root, root_sah, root_tris, N, ci, ct, sched,
+                                             rounds, parent, left, right, bbmin, bbmax, area,
+                                             nsah, span);
+                    else
+                        trbvh_treelet<true>(lds, hl, root, root_sah, root_tris, N, ci, ct, sched,
+                                            rounds, parent, left, right, bbmin, bbmax, area,
+                                            nsah, span);
+                }
             }
         }
         if (cur >= 0) cur = parent[cur];
@@ -561,7 +618,7 @@ atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma
     return launch(ctx, "atrbvh_pass", 0.0, [&] {
         k_treelet_pass<true><<<div_up(N, 256), 256, lds, ctx->stream>>>(
             N, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax, t->area,
-            tris, nsah, counters, nullptr, 0, half_words);
+            tris, nsah, counters, t->span, nullptr, 0, half_words);
     });
 }
 
@@ -575,7 +632,7 @@ atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, fl
     return launch(ctx, "trbvh_pass", 0.0, [&] {
         k_treelet_pass<false><<<div_up(N, 256), 256, lds, ctx->stream>>>(
             N, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax, t->area,
-            tris, nsah, counters, d_schedule, rounds, kTrWords);
+            tris, nsah, counters, t->span, d_schedule, rounds, kTrWords);
     });
 }
 

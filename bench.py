@@ -1,0 +1,271 @@
+"""Benchmark: ATRBVH build throughput on one MI355X.
+
+Workload (the largest single-GPU configuration of the workload set):
+2,097,152 fp32 triangles from a 64-cluster Gaussian mixture (centres uniform
+in [0,1]^3, per-cluster sigma log-uniform in [0.005, 0.1], equilateral
+triangles with log-normal edge length, median 0.002, sigma 0.7, random
+rotation).  One step = LBVH with 63-bit Morton keys -> ATRBVH (treelet 9,
+2 iterations) -> SAH leaf collapse (ci 1.2, ct 1), with the scene resident in
+HBM.  value = triangles / step time (Mtris/s).
+
+Also reported: SAH after each stage, the TRBVH (treelet 7, 3 iterations)
+build, traversal Mrays/s for 4,194,304 primary rays (2048x2048 pinhole at
+(0.5, 0.5, -1) looking +z, 90 degree field of view) and 4,194,304 diffuse
+rays from the primary hits, a per-kernel HIP-event breakdown, the roofline
+of the dominant streaming kernel, and the CPU oracle timed on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import atrbvh_amd as ab  # noqa: E402
+
+N_TRIS = 2_097_152
+SEED = 2
+GRID = 2048
+EYE = (0.5, 0.5, -1.0)
+BASIS = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+HBM_PEAK_GBS = 8000.0
+METRIC = "ATRBVH build Mtris/s & SAH at 1M tris; batched builds/s vs plain; Mrays/s"
+# streaming kernels priced against HBM bandwidth
+STREAMING = ("encode63", "onesweep64", "build_refit63")
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_step(scene, treelet="atrbvh", want_sah=False):
+    tree = ab.LBVHBuilder(True).BuildTree(scene)
+    if treelet == "atrbvh":
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(tree)
+    elif treelet == "trbvh":
+        ab.TRBVHOptimizer(7, 3).Optimize(tree)
+    collapsed, sah = ab.BVHTreeCollapser().Collapse(tree, want_sah=want_sah)
+    return tree, collapsed, sah
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the roofline kernel measured with rocprofv3 PMC
+    counters (committed summary), or None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", "onesweep64_c2_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(sample_tris: int):
+    """The oracle (serial C restatement of the reference build path) on a
+    bounded sample of the same workload, on one host core."""
+    import oracle
+
+    v = ab.gen_gmm(SEED, sample_tris)
+    mn, mx = oracle.scene_bbox(v)
+    t0 = time.perf_counter()
+    t = oracle.lbvh(v, mn, mx, 63)
+    oracle.atrbvh(t, 9, 2)
+    oracle.collapse(t)
+    dt = time.perf_counter() - t0
+    return {"value": round(sample_tris / dt / 1e6, 4), "unit": "Mtris/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle LBVH63 + ATRBVH(9,2) + collapse of a {sample_tris:,}-triangle "
+                      f"GMM scene from the same generator, single thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-tris", type=int, default=N_TRIS)
+    ap.add_argument("--no-trace", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU "
+              "sharding of this path is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"[bench] generating {N_TRIS:,}-triangle GMM scene")
+    v = ab.gen_gmm(SEED, N_TRIS)
+    ctx = ab.Context(0)
+    scene = ctx.scene(v)
+
+    # ---------------------------------------------------------------- timed build
+    for _ in range(args.warmup):
+        t, c, _ = build_step(scene)
+        c.close()
+        t.close()
+    ctx.sync()
+    log(f"[bench] warmup done ({args.warmup} steps)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t, c, _ = build_step(scene)
+        c.close()
+        t.close()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    mtris = N_TRIS / (ms_per_step * 1e-3) / 1e6
+    log(f"[bench] build {ms_per_step:.3f} ms/step, {mtris:.1f} Mtris/s")
+
+    # ---------------------------------------------------------------- profiled steps
+    ctx.profile(True)
+    ctx.profile_reset()
+    prof_steps = max(3, min(args.steps, 10))
+    for _ in range(prof_steps):
+        t, c, _ = build_step(scene)
+        c.close()
+        t.close()
+    ctx.sync()
+    recs = ctx.profile_records()
+    ctx.profile(False)
+    kernels = {}
+    for name, r in recs.items():
+        per = r["launches"] / prof_steps
+        kernels[name] = {"ms_per_step": round(r["ms"] / prof_steps, 4),
+                         "launches_per_step": per,
+                         "avg_launch_us": round(r["ms"] * 1000.0 / max(r["launches"], 1), 3)}
+        if r["bytes"] > 0 and r["ms"] > 0:
+            kernels[name]["GB_s"] = round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)
+    gpu_ms = sum(r["ms"] for r in recs.values()) / prof_steps
+    streaming = {k: recs[k] for k in STREAMING if k in recs}
+    s_bytes = sum(r["bytes"] for r in streaming.values()) / prof_steps
+    s_ms = sum(r["ms"] for r in streaming.values()) / prof_steps
+    dom = max(streaming, key=lambda k: streaming[k]["ms"])
+    dr = recs[dom]
+    per_launch_bytes = dr["bytes"] / dr["launches"]
+    per_launch_s = dr["ms"] * 1e-3 / dr["launches"]
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(),
+                "algorithmic_bytes_per_launch": per_launch_bytes,
+                "avg_launch_us": round(per_launch_s * 1e6, 3),
+                "streaming_group": {"kernels": list(streaming), "bytes_per_step": s_bytes,
+                                    "ms_per_step": round(s_ms, 4),
+                                    "GB_s": round(s_bytes / (s_ms * 1e-3) / 1e9, 1),
+                                    "frac": round(s_bytes / (s_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                                  4)}}
+    log(f"[bench] profiled: gpu {gpu_ms:.3f} ms/step; dominant streaming kernel {dom} "
+        f"{achieved:.0f} GB/s")
+
+    # ---------------------------------------------------------------- SAH per stage
+    tree = ab.LBVHBuilder(True).BuildTree(scene)
+    sah_lbvh = tree.SAH()
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(tree)
+    sah_atrbvh = tree.SAH()
+    collapsed, sah_collapsed = ab.BVHTreeCollapser().Collapse(tree)
+    stats = collapsed.stats()
+
+    # ---------------------------------------------------------------- TRBVH build
+    ctx.sync()
+    tt0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        t2, c2, _ = build_step(scene, "trbvh")
+        c2.close()
+        t2.close()
+    ctx.sync()
+    trbvh_ms = (time.perf_counter() - tt0) * 1000.0 / reps
+    t2 = ab.LBVHBuilder(True).BuildTree(scene)
+    ab.TRBVHOptimizer(7, 3).Optimize(t2)
+    sah_trbvh = t2.SAH()
+    c2, sah_trbvh_collapsed = ab.BVHTreeCollapser().Collapse(t2)
+    c2.close()
+    t2.close()
+    log(f"[bench] SAH lbvh {sah_lbvh:.3f} atrbvh {sah_atrbvh:.3f} collapsed {sah_collapsed:.3f} "
+        f"trbvh {sah_trbvh:.3f}; trbvh build {trbvh_ms:.2f} ms")
+
+    # ---------------------------------------------------------------- traversal
+    trace = None
+    if not args.no_trace:
+        r = GRID * GRID
+        d_prim = ctx.buffer(r * 32)
+        d_res = ctx.buffer(r * 16)
+        d_sec = ctx.buffer(r * 32)
+        d_res2 = ctx.buffer(r * 16)
+        ctx.rays_pinhole(EYE, BASIS, 1.0, GRID, GRID, float("inf"), d_prim)
+        collapsed.trace(d_prim, r, d_res)
+        ctx.rays_diffuse(scene, d_prim, d_res, r, 7, float("inf"), d_sec)
+        ctx.sync()
+        res = d_res.download(np.int32, (r, 4))
+        n_diffuse = int((res[:, 0] >= 0).sum())
+        treps = 5
+        ctx.sync()
+        w0 = time.perf_counter()
+        for _ in range(treps):
+            collapsed.trace(d_prim, r, d_res)
+        ctx.sync()
+        w1 = time.perf_counter()
+        for _ in range(treps):
+            collapsed.trace(d_sec, r, d_res2)
+        ctx.sync()
+        w2 = time.perf_counter()
+        prim_ms = (w1 - w0) * 1000.0 / treps
+        sec_ms = (w2 - w1) * 1000.0 / treps
+        trace = {"primary_rays": r, "primary_ms": round(prim_ms, 3),
+                 "primary_Mrays_s": round(r / (prim_ms * 1e-3) / 1e6, 1),
+                 "primary_hit_rate": round(n_diffuse / r, 4),
+                 "diffuse_rays": n_diffuse, "diffuse_ms": round(sec_ms, 3),
+                 "diffuse_Mrays_s": round(n_diffuse / (sec_ms * 1e-3) / 1e6, 1)}
+        log(f"[bench] trace primary {trace['primary_Mrays_s']} Mrays/s, diffuse "
+            f"{trace['diffuse_Mrays_s']} Mrays/s")
+    collapsed.close()
+    tree.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline on {args.cpu_tris:,} triangles")
+        cpu = cpu_baseline(args.cpu_tris)
+
+    out = {
+        "metric": METRIC,
+        "value": round(mtris, 2),
+        "unit": "Mtris/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "2,097,152-triangle 64-cluster GMM soup; LBVH 63-bit Morton + "
+                               "ATRBVH (treelet 9, 2 iterations) + SAH collapse (ci 1.2, ct 1)",
+                   "triangles": N_TRIS, "morton_bits": 63, "treelet": 9, "iterations": 2,
+                   "seed": SEED},
+        "sah": {"lbvh": round(sah_lbvh, 4), "atrbvh": round(sah_atrbvh, 4),
+                "atrbvh_collapsed": round(sah_collapsed, 4), "trbvh": round(sah_trbvh, 4),
+                "trbvh_collapsed": round(sah_trbvh_collapsed, 4)},
+        "tree_height": stats["height"],
+        "trbvh_build": {"ms": round(trbvh_ms, 3),
+                        "Mtris_s": round(N_TRIS / (trbvh_ms * 1e-3) / 1e6, 2)},
+        "trace": trace,
+        "gpu_ms_per_step": round(gpu_ms, 4),
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    scene.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

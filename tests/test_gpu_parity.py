@@ -201,3 +201,82 @@ def test_trace_any_hit(ctx):
     c.close()
     t.close()
     s.close()
+
+
+# ------------------------------------------------------------------ full size
+def test_c2_full_pipeline_bit_exact(ctx):
+    """2,097,152-triangle GMM scene: every kernel runs with thousands of
+    workgroups, so nodes that cross workgroups (cross-XCD hand-over) are
+    exercised at the real shape."""
+    n = 2_097_152
+    v = ab.gen_gmm(2, n)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(True).BuildTree(s)
+    lb = t.download()
+    ref = oracle.lbvh(v, mn, mx, 63)
+    assert_tree_equal(lb, tree_arrays(ref), "c2.lbvh")
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+    dev = t.download()
+    oracle.atrbvh(ref, 9, 2)
+    assert_tree_equal(dev, tree_arrays(ref), "c2.atrbvh")
+    c, sah = ab.BVHTreeCollapser().Collapse(t)
+    rc, rsah = oracle.collapse(ref)
+    assert bits(np.float32(sah)) == bits(np.float32(rsah))
+    cd = c.download()
+    for k in ("data", "count"):
+        assert np.array_equal(cd[k], getattr(rc, k)), k
+    assert np.array_equal(cd["tri_index"][: rc.used], rc.tri_index[: rc.used])
+    c.close()
+    t.close()
+    s.close()
+
+
+def test_trbvh_large_bit_exact(ctx):
+    n = 300_000
+    v = ab.gen_gmm(9, n)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(True).BuildTree(s)
+    ab.TRBVHOptimizer(7, 3).Optimize(t)
+    ref = oracle.lbvh(v, mn, mx, 63)
+    oracle.trbvh(ref, 7, 3)
+    assert_tree_equal(t.download(), tree_arrays(ref), "trbvh300k")
+    t.close()
+    s.close()
+
+
+def test_repeat_builds_identical(ctx):
+    v = ab.gen_gmm(3, 500_000)
+    s = ctx.scene(v)
+    outs = []
+    for _ in range(3):
+        t = ab.LBVHBuilder(True).BuildTree(s)
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+        outs.append(t.download())
+        t.close()
+    for o in outs[1:]:
+        assert_tree_equal(o, outs[0], "repeat")
+    s.close()
+
+
+def test_refit_matches_rebuild_boxes(ctx):
+    """Refit over a fixed topology after moving vertices equals a recursive
+    union of the new leaf boxes (checked against the oracle refit)."""
+    n = 200_000
+    v = ab.gen_soup(21, n)
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(False).BuildTree(s)
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+    topo = t.download()
+    v2 = v.copy()
+    v2[:, :3] += np.float32(0.01) * np.sin(np.arange(v2.shape[0], dtype=np.float32))[:, None]
+    s2 = ctx.scene(v2, mn, mx)
+    t.refit(s2)
+    dev = t.download()
+    ref = oracle.Tree(n)
+    for k in ("parent", "left", "right", "data"):
+        getattr(ref, k)[...] = topo[k]
+    oracle.refit(v2, ref)
+    assert_tree_equal(dev, tree_arrays(ref), "refit")
+    t.close()
+    s.close()
+    s2.close()
