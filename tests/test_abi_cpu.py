@@ -1,0 +1,98 @@
+"""CPU checks of the C-ABI library: it loads without a GPU, exports every
+symbol the public headers declare, validates arguments before touching the
+device, and its host utilities (scene generators, lane schedule) behave."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import atrbvh_amd as ab
+import oracle
+
+HDRS = [os.path.join(ab.INCLUDE_DIR, h) for h in ("atrbvh.h", "atrbvh_scenes.h")]
+
+
+def declared_symbols():
+    names = set()
+    for h in HDRS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(atr_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(ab.LIB_PATH)
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(declared_symbols()) >= 40
+
+
+def test_oracle_library_loads():
+    assert oracle.lib().orc_expand_bits10(1) == 1
+
+
+def test_version_string():
+    assert b"gfx950" in ab.lib().atr_version()
+
+
+def _gpu_present():
+    import subprocess
+    try:
+        out = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        return False
+    return "gfx9" in out
+
+
+def test_argument_validation_without_device():
+    L = ab.lib()
+    out = ctypes.c_void_p()
+    assert L.atr_context_create(0, None) == 1
+    assert L.atr_lbvh_build(None, 30, ctypes.byref(out)) == 1
+    assert L.atr_trbvh_optimize(None, 7, 3, 1.2, 1.0) == 1
+    assert L.atr_atrbvh_optimize(None, 9, 2, 1.2, 1.0) == 1
+    assert L.atr_collapse(None, 1.2, 1.0, ctypes.byref(out), None) == 1
+    assert L.atr_trace(None, None, 0, 0, None) == 1
+    assert L.atr_sync(None) == 1
+    assert L.atr_scene_create(None, None, 10, None, None, ctypes.byref(out)) == 1
+    assert L.atr_sort_pairs(None, 32, None, None, 0) == 1
+    assert b"NULL" in L.atr_last_error() or len(L.atr_last_error()) > 0
+    # destroy of NULL handles is a no-op
+    assert L.atr_tree_destroy(None) == 0 and L.atr_scene_destroy(None) == 0
+
+
+def test_context_without_gpu_reports_device_error():
+    if _gpu_present():
+        pytest.skip("a GPU is present")
+    with pytest.raises(ab.AtrError) as e:
+        ab.Context(0)
+    assert e.value.status == 3
+
+
+def test_scene_generators_are_deterministic_and_shaped():
+    a = ab.gen_soup(1, 1000)
+    b = ab.gen_soup(1, 1000)
+    c = ab.gen_soup(2, 1000)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    t = a.reshape(-1, 3, 4)
+    cen = t[:, :, :3].mean(axis=1)
+    assert np.all(t[:, :, 3] == 0)
+    assert 0.45 < cen.mean() < 0.55
+    dev = (t[:, :, :3] - cen[:, None, :]).std()
+    assert 0.003 < dev < 0.006
+
+    g = ab.gen_gmm(2, 20000).reshape(-1, 3, 4)[:, :, :3]
+    e = np.linalg.norm(g[:, 1] - g[:, 0], axis=1)
+    assert 0.0015 < np.median(e) < 0.0025  # median edge 0.002
+    e2 = np.linalg.norm(g[:, 2] - g[:, 1], axis=1)
+    assert np.allclose(e, e2, rtol=1e-3, atol=1e-7)  # equilateral
+
+
+def test_library_schedule_equals_oracle():
+    for n in (3, 4, 5, 6, 7):
+        assert np.array_equal(ab.trbvh_schedule(n, 32), oracle.trbvh_schedule(n, 32))
