@@ -96,6 +96,7 @@ def lib():
                                           ctypes.POINTER(ctypes.c_uint32)]
         L.atr_tree_sah.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _f32p]
         L.atr_tree_num_triangles.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.atr_collapsed_num_triangles.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32)]
         L.atr_tree_download.argtypes = [_vp, ctypes.POINTER(TreeView)]
         L.atr_tree_device_view.argtypes = [_vp, ctypes.POINTER(TreeView)]
         L.atr_tree_upload.argtypes = [_vp, ctypes.POINTER(TreeView)]

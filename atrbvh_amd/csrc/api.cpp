@@ -541,6 +541,13 @@ atr_status atr_tree_num_triangles(const atr_tree* t, uint32_t* n)
     return ATR_OK;
 }
 
+atr_status atr_collapsed_num_triangles(const atr_collapsed* c, uint32_t* n)
+{
+    if (!c || !n) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *n = c->n;
+    return ATR_OK;
+}
+
 atr_status atr_tree_device_view(atr_tree* t, atr_tree_view* v)
 {
     if (!t || !v) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");

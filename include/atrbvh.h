@@ -139,8 +139,9 @@ atr_status atr_atrbvh_optimize(atr_tree* tree, int treelet_size, int iterations,
 /* BVHTreeCollapser::Collapse (BVHRT-Core/src/BVHTreeCollapser.h:39,
  * BVHTreeCollapser.cpp:21-93).  *sah (nullable) = cost[root]/area[root]; when
  * sah is non-NULL the call blocks until it is known (as the reference does),
- * otherwise it is asynchronous and atr_collapsed_stats gives the SAH later.  When the tree is bound to a scene the collapsed tree also carries
- * the traversal layout atr_trace needs. */
+ * otherwise it is asynchronous and atr_collapsed_stats gives the SAH later.
+ * When the tree is bound to a scene the collapsed tree also carries the
+ * traversal layout atr_trace needs. */
 atr_status atr_collapse(atr_tree* tree, float ci, float ct, atr_collapsed** out, float* sah);
 
 /* SoABVHTree::SAH (BVHRT-Core/src/SoABVHTree.cpp:50-74), same summation order;
@@ -165,6 +166,7 @@ atr_status atr_tree_bind_scene(atr_tree* tree, atr_scene* scene);
 
 /* DeviceToHostCollapsedTree / FreeDeviceCollapsedTree
  * (BVHRT-Core/src/BVHTreeInstanceManager.cpp:196-270). Blocking. */
+atr_status atr_collapsed_num_triangles(const atr_collapsed* c, uint32_t* n);
 atr_status atr_collapsed_download(atr_collapsed* c, atr_collapsed_view* host);
 /* SAH (cost[root]/area[root], BVHTreeCollapser.cpp:69-74), tree height and the
  * number of triangles placed in leaf lists.  Blocking. */

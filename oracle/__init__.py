@@ -356,6 +356,18 @@ def trbvh_dp(leaf_cost, subset_area, treelet_tris, ci=1.2, ct=1.0):
     return float(v), costs, masks
 
 
+def find_minimum(D, last_row: int) -> int:
+    d = _f32(np.zeros(496, np.float32))
+    src = _f32(D)
+    d[: src.size] = src
+    return lib().orc_find_minimum(_p(d, _f32p), last_row)
+
+
+def find_leaf_to_expand(areas) -> int:
+    a = _f32(areas)
+    return lib().orc_find_leaf_to_expand(a.size, _p(a, _f32p))
+
+
 def atrbvh_cluster(boxes, leaf_sah, ci=1.2):
     n = len(leaf_sah)
     b = _f32(boxes).reshape(n, 6)

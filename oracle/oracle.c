@@ -842,6 +842,15 @@ static int emulate_find_minimum(const float* D, int lastRow)
     return mi[0];
 }
 
+/* Test entry points for the two emulated reductions above. */
+int orc_find_minimum(const float* D, int lastRow) { return emulate_find_minimum(D, lastRow); }
+
+int orc_find_leaf_to_expand(int m, const float* area)
+{
+    if (m < 1 || m > 32) return -1;
+    return emulate_find_leaf_to_expand(m, area);
+}
+
 /* Agglomerative clustering of one treelet (AgglomerativeTreelet.cu:492-532 with
  * calculateDistancesMatrix :18-63, updateState :101-150 and
  * updateDistancesMatrix :152-188).  Lane state: node/sah/box per slot.

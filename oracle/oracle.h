@@ -21,6 +21,11 @@
  *   atrbvh sched  pair coverage (each i>j pair exactly once)
  *   atrbvh        O(n^3) greedy agglomerative loop with the tie rule, on
  *                 synthetic treelets; tree invariants after optimisation
+ *   tie order     closed-form tie rules of find_minimum, find_leaf_to_expand
+ *                 and the DP partition choice, on planted ties
+ *   parity unpinned: TRBVH rewrite slot order and the reference quirks that
+ *                 feed later decisions (right-child cost, root nodesSah not
+ *                 updated, root triangle count in every subset's DP term)
  *   collapse      recursive cost / leaf-set partition / SAH=cost/area
  *   trace         brute-force Moller-Trumbore over all triangles
  */
@@ -106,6 +111,12 @@ int orc_atrbvh_optimize(orc_tree* t, int treelet, int iterations, float ci, floa
 /* Single-treelet primitives exposed for the closed-form tests. */
 float orc_trbvh_dp(int n, const float* leaf_cost, const float* subset_area, int treelet_tris,
                    float ci, float ct, float* costs_out, int32_t* masks_out);
+/* The shuffle-reduction emulations used by the treelet passes, exposed for
+ * their tests: findMinimumDistance over the first lastRow*(lastRow+1)/2
+ * entries of D (AgglomerativeTreelet.cu:65-99) and findLeafToExpand over m
+ * candidate areas (Treelet.cuh:23-40).  Return the winning index. */
+int orc_find_minimum(const float* D, int lastRow);
+int orc_find_leaf_to_expand(int m, const float* area);
 int orc_atrbvh_cluster(int n, const float* boxes /* n*6 */, const float* leaf_sah, float ci,
                        int32_t* merges /* (n-1)*2 (row, col) */, float* areas_out /* n */);
 

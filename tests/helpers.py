@@ -112,3 +112,53 @@ def compare_hits(dev: np.ndarray, ref: np.ndarray, rtol: float = 1e-6):
     h = hit_r
     assert np.allclose(td[h], tr[h], rtol=rtol, atol=0)
     return int(np.count_nonzero(~same))
+
+
+COMPAT_SRC = "tests/compat_main.cpp"
+COMPAT_BIN = "tests/_bin/compat_main"
+
+
+def build_compat_driver(root: str) -> str:
+    """Compile tests/compat_main.cpp (a host program using the C++ facade
+    include/bvhrt_compat.hpp) against libatrbvh.so with g++."""
+    import os
+    import subprocess
+
+    out = os.path.join(root, COMPAT_BIN)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    libdir = os.path.join(root, "atrbvh_amd")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-I" + os.path.join(root, "include"), os.path.join(root, COMPAT_SRC),
+           "-L" + libdir, "-latrbvh", "-Wl,-rpath,$ORIGIN/../../atrbvh_amd", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def run_compat_driver(root: str, v: np.ndarray, morton64: bool, opt: str, tmpdir: str):
+    """Runs the facade driver on vertices v; returns (returncode, stdout,
+    parsed results or None)."""
+    import os
+    import subprocess
+
+    exe = os.path.join(root, COMPAT_BIN)
+    n = v.size // 12
+    vin = os.path.join(tmpdir, "v.f32")
+    out = os.path.join(tmpdir, "out.bin")
+    np.ascontiguousarray(v, np.float32).tofile(vin)
+    r = subprocess.run([exe, vin, str(n), "1" if morton64 else "0", opt, out],
+                       capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return r.returncode, r.stdout, None
+    raw = np.fromfile(out, np.uint8)
+    head = raw[:12]
+    sah_tree, sah_c = head[:8].view(np.float32)
+    used = int(head[8:12].view(np.uint32)[0])
+    m = 2 * n - 1
+    ints = raw[12:].view(np.int32)
+    res = {"sah_tree": sah_tree, "sah_collapsed": sah_c, "used": used}
+    for i, k in enumerate(("parent", "left", "right", "data", "c_data", "c_count")):
+        res[k] = ints[i * m:(i + 1) * m]
+    res["tri_index"] = ints[6 * m:6 * m + used]
+    return 0, r.stdout, res

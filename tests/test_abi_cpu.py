@@ -96,3 +96,18 @@ def test_scene_generators_are_deterministic_and_shaped():
 def test_library_schedule_equals_oracle():
     for n in (3, 4, 5, 6, 7):
         assert np.array_equal(ab.trbvh_schedule(n, 32), oracle.trbvh_schedule(n, 32))
+
+
+def test_cpp_facade_compiles_and_reports_errors(tmp_path):
+    """include/bvhrt_compat.hpp builds warning-free under -Wall -Wextra -Werror
+    and turns a library failure into BVHRT::Error (without a GPU: the device
+    error of context creation)."""
+    from helpers import build_compat_driver, run_compat_driver
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    build_compat_driver(root)
+    if _gpu_present():
+        pytest.skip("a GPU is present")
+    rc, out, res = run_compat_driver(root, ab.gen_soup(1, 64), False, "atrbvh", str(tmp_path))
+    assert rc == 3 and res is None
+    assert out.startswith("status 3:")

@@ -280,3 +280,35 @@ def test_refit_matches_rebuild_boxes(ctx):
     t.close()
     s.close()
     s2.close()
+
+
+# ------------------------------------------------------------------ C++ facade
+@pytest.mark.parametrize("opt,morton64", [("atrbvh", False), ("trbvh", True), ("none", False)])
+def test_cpp_facade_matches_oracle(tmp_path, opt, morton64):
+    """The Renderer::getCudaBVH call sequence through include/bvhrt_compat.hpp
+    (a separate host process on the C ABI) gives the oracle's trees."""
+    import os
+
+    from helpers import COMPAT_BIN, run_compat_driver
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if not os.path.exists(os.path.join(root, COMPAT_BIN)):
+        pytest.fail(f"{COMPAT_BIN} is not built (the build step compiles it)")
+    n = 30000
+    v = ab.gen_soup(91, n)
+    rc, out, res = run_compat_driver(root, v, morton64, opt, str(tmp_path))
+    assert rc == 0, out
+    mn, mx = oracle.scene_bbox(v)
+    ref = oracle.lbvh(v, mn, mx, 63 if morton64 else 30)
+    if opt == "atrbvh":
+        oracle.atrbvh(ref, 9, 2)
+    elif opt == "trbvh":
+        oracle.trbvh(ref, 7, 3)
+    for k in ("parent", "left", "right", "data"):
+        assert np.array_equal(res[k], getattr(ref, k)), k
+    assert bits(res["sah_tree"]) == bits(oracle.sah(ref))
+    c, csah = oracle.collapse(ref)
+    assert bits(res["sah_collapsed"]) == bits(csah)
+    assert res["used"] == c.used
+    assert np.array_equal(res["c_data"], c.data) and np.array_equal(res["c_count"], c.count)
+    assert np.array_equal(res["tri_index"], c.tri_index[: c.used])

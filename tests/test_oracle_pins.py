@@ -490,3 +490,109 @@ def test_trace_equals_brute_force():
     extra[:, 7] = np.inf
     rays = np.concatenate([rays, extra])
     compare_hits(oracle.trace(c, v, rays), oracle.trace_brute(v, rays))
+
+
+# ------------------------------------------------------------------ tie order of the emulated reductions
+def _bitrev(x: int, bits: int) -> int:
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def test_find_minimum_tie_order_closed_form():
+    """findMinimum (AgglomerativeTreelet.cu:65-99): lane l keeps the first
+    strict minimum of entries l, l+32, ...; the 32-lane halving tree keeps its
+    own value on ties, so among lanes holding the minimum the one with the
+    smallest 5-bit bit-reversed index wins.  Predicted from that rule alone
+    for planted ties."""
+    rng = np.random.default_rng(21)
+    for _ in range(3000):
+        L = int(rng.integers(1, 32))
+        size = L * (L + 1) // 2
+        D = (1.0 + rng.random(size)).astype(np.float32)
+        k = int(rng.integers(1, min(size, 8) + 1))
+        S = sorted(rng.choice(size, k, replace=False).tolist())
+        D[S] = np.float32(0.5)
+        first_per_lane = {}
+        for p in S:
+            first_per_lane.setdefault(p % 32, p)
+        lane = min(first_per_lane, key=lambda l: _bitrev(l, 5))
+        assert oracle.find_minimum(D, L) == first_per_lane[lane], (L, S)
+
+
+def test_find_leaf_to_expand_tie_order():
+    """findLeafToExpand (Treelet.cuh:23-40) over m candidates, strict '>':
+    a unique maximum always wins; for power-of-two m the halving tree gives
+    ties to the smallest log2(m)-bit bit-reversed index; for any m lane 0 wins
+    every tie it takes part in and a tie is always won by a tied lane."""
+    rng = np.random.default_rng(22)
+    for m in range(1, 33):
+        for _ in range(300):
+            a = rng.random(m).astype(np.float32)
+            assert oracle.find_leaf_to_expand(a) == int(np.argmax(a))
+            k = int(rng.integers(2, m + 1)) if m > 1 else 1
+            S = sorted(rng.choice(m, k, replace=False).tolist())
+            a[S] = np.float32(2.0)
+            w = oracle.find_leaf_to_expand(a)
+            assert w in S
+            if 0 in S:
+                assert w == 0
+            if m & (m - 1) == 0:
+                b = m.bit_length() - 1
+                assert w == min(S, key=lambda i: _bitrev(i, b))
+
+
+def _submasks_ascending(mask: int):
+    return [p for p in range(1, mask + 1) if p & mask == p]
+
+
+@pytest.mark.parametrize("pattern", ["popcount", "random_int"])
+def test_trbvh_dp_tie_order_closed_form(pattern):
+    """TRBVH DP partition choice under ties (TRBVH.cu:100-220).  For every
+    subset S the candidates are the submasks P of S without S's lowest element
+    (cost[P] + cost[S^P]).  Sizes 2..n-2: sequential scan in ascending P keeps
+    the first strict minimum.  Size n-1: base b (the b-th such P in ascending
+    order) is scanned by lane (b-1) mod 4, ties between lanes go to the
+    2-bit bit-reversed order 0, 2, 1, 3.  Size n: P = 2(l+1) + 64j is scanned
+    by lane l, ties go to the smallest 5-bit bit-reversed lane.  Inputs are
+    small integers so that many candidates tie exactly."""
+    rng = np.random.default_rng(23 if pattern == "popcount" else 24)
+    n = 7
+    full = (1 << n) - 1
+    for trial in range(40):
+        leaf = np.ones(n, np.float32) if pattern == "popcount" else \
+            rng.integers(1, 4, n).astype(np.float32)
+        if pattern == "popcount":
+            area = np.array([bin(s).count("1") for s in range(1 << n)], np.float32)
+        else:
+            area = rng.integers(1, 6, 1 << n).astype(np.float32)
+        T = int(rng.integers(7, 40))
+        _, costs, masks = oracle.trbvh_dp(leaf, area, T, ci=1.0 + trial % 3, ct=1.0)
+        ties_seen = 0
+        for s in range(1, 1 << n):
+            size = bin(s).count("1")
+            if size < 2:
+                continue
+            low = s & -s
+            cands = _submasks_ascending(s & ~low)
+            vals = [np.float32(costs[p] + costs[s ^ p]) for p in cands]
+            best = min(vals)
+            tied = [p for p, v in zip(cands, vals) if v == best]
+            ties_seen += len(tied) > 1
+            if size <= n - 2:
+                want = tied[0]
+            elif size == n - 1:
+                by_lane = {}
+                for b, p in enumerate(cands, start=1):
+                    if p in tied:
+                        by_lane.setdefault((b - 1) % 4, p)
+                lane = min(by_lane, key=lambda r: _bitrev(r, 2))
+                want = by_lane[lane]
+            else:
+                by_lane = {}
+                for p in cands:  # all even p, ascending
+                    if p in tied:
+                        by_lane.setdefault((p // 2 - 1) % 32, p)
+                lane = min(by_lane, key=lambda r: _bitrev(r, 5))
+                want = by_lane[lane]
+            assert (int(masks[s]) & 0xFF) == want, (trial, s, size, tied)
+        assert ties_seen > 0
+    assert full == 127
