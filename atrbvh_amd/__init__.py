@@ -24,6 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libatrbvh.so")
 INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
 
+_u32p = ctypes.POINTER(ctypes.c_uint32)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f32p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
@@ -122,6 +123,21 @@ def lib():
                                   _f32p]
         L.atr_gen_gmm.restype = None
         L.atr_trbvh_schedule.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int]
+        L.atr_gen_batch_sizes.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32, _u32p]
+        L.atr_gen_batch_sizes.restype = None
+        L.atr_gen_soup_batch.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _u32p, ctypes.c_float,
+                                         _f32p]
+        L.atr_gen_soup_batch.restype = None
+        L.atr_batch_create.argtypes = [_vp, _vp, _u32p, ctypes.c_uint32, _vp,
+                                       ctypes.POINTER(_vp)]
+        L.atr_batch_create_device.argtypes = L.atr_batch_create.argtypes
+        L.atr_batch_vertices.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.atr_batch_build.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_int]
+        L.atr_batch_sah.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _f32p]
+        L.atr_batch_extract.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.atr_batch_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -146,6 +162,21 @@ def gen_gmm(seed: int, n: int, clusters: int = 64, sigma_lo: float = 0.005, sigm
             edge_median: float = 0.002, edge_sigma: float = 0.7) -> np.ndarray:
     v = np.zeros((3 * n, 4), np.float32)
     lib().atr_gen_gmm(seed, n, clusters, sigma_lo, sigma_hi, edge_median, edge_sigma, _ptr(v))
+    return v
+
+
+def gen_batch_sizes(seed: int, meshes: int, lo: int = 64, hi: int = 65536) -> np.ndarray:
+    """Prefix offsets (meshes + 1) of log-uniform mesh sizes in [lo, hi]."""
+    off = np.zeros(meshes + 1, np.uint32)
+    lib().atr_gen_batch_sizes(seed, meshes, lo, hi, _ptr(off, _u32p))
+    return off
+
+
+def gen_soup_batch(seed_base: int, offsets: np.ndarray, sigma: float = 0.005) -> np.ndarray:
+    """Mesh m = gen_soup(seed_base + m, N_m, sigma), concatenated mesh-major."""
+    off = np.ascontiguousarray(offsets, np.uint32)
+    v = np.zeros((3 * int(off[-1]), 4), np.float32)
+    lib().atr_gen_soup_batch(seed_base, off.size - 1, _ptr(off, _u32p), sigma, _ptr(v))
     return v
 
 
@@ -285,6 +316,17 @@ class Scene:
                                          mx.ctypes.data, ctypes.byref(h)))
         self.h = h.value
 
+    @classmethod
+    def from_device(cls, ctx: Context, d_vertices: int, n: int) -> "Scene":
+        """Scene from float4[3n] already on the device (copied device-to-device)."""
+        s = cls.__new__(cls)
+        s.ctx = ctx
+        s.n = int(n)
+        h = _vp()
+        check(lib().atr_scene_create_device(ctx.h, d_vertices, s.n, None, None, ctypes.byref(h)))
+        s.h = h.value
+        return s
+
     def bounds(self):
         mn = np.zeros(3, np.float32)
         mx = np.zeros(3, np.float32)
@@ -393,6 +435,51 @@ class Collapsed:
     def close(self):
         if self.h:
             lib().atr_collapsed_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Many independent meshes built as one forest in a single launch sequence
+    (optionally a captured HIP graph); each mesh's tree equals the one
+    LBVHBuilder(False) + AgglomerativeTreeletOptimizer gives for it alone."""
+
+    def __init__(self, ctx: Context, vertices: np.ndarray, offsets: np.ndarray, bboxes=None):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 4)
+        self.offsets = np.ascontiguousarray(offsets, np.uint32)
+        self.meshes = self.offsets.size - 1
+        assert v.shape[0] == 3 * int(self.offsets[-1])
+        self.ctx = ctx
+        bb = None
+        if bboxes is not None:
+            bb = np.ascontiguousarray(bboxes, np.float32).reshape(self.meshes, 6)
+        h = _vp()
+        check(lib().atr_batch_create(ctx.h, v.ctypes.data, _ptr(self.offsets, _u32p),
+                                     self.meshes, bb.ctypes.data if bb is not None else None,
+                                     ctypes.byref(h)))
+        self.h = h.value
+
+    def build(self, treelet: int = 9, iterations: int = 2, ci: float = 1.2, ct: float = 1.0,
+              graph: bool = True):
+        check(lib().atr_batch_build(self.h, treelet, iterations, ci, ct, 1 if graph else 0))
+
+    def sah(self, ci: float = 1.2, ct: float = 1.0) -> np.ndarray:
+        out = np.zeros(self.meshes, np.float32)
+        check(lib().atr_batch_sah(self.h, ci, ct, _ptr(out)))
+        return out
+
+    def extract(self, mesh: int) -> Tree:
+        h = _vp()
+        check(lib().atr_batch_extract(self.h, mesh, ctypes.byref(h)))
+        return Tree(self.ctx, h.value, int(self.offsets[mesh + 1] - self.offsets[mesh]))
+
+    def device_vertices(self) -> int:
+        p = _vp()
+        check(lib().atr_batch_vertices(self.h, ctypes.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.h:
+            lib().atr_batch_destroy(self.h)
             self.h = None
 
 

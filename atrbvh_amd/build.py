@@ -39,8 +39,8 @@ DEVICE = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "api.cpp", "scenes.cpp",
-           "schedule.cpp"]
+SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "batch.hip", "api.cpp",
+           "scenes.cpp", "schedule.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -65,7 +65,7 @@ def build(jobs: int | None = None) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    cmd = [HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

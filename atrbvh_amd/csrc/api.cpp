@@ -82,7 +82,7 @@ static hipEvent_t take_event(atr_context* ctx)
 
 LaunchScope::LaunchScope(atr_context* c, const char* name, double bytes) : ctx(c), rec(-1)
 {
-    if (!ctx->profiling) return;
+    if (!ctx->profiling || ctx->capturing) return;
     auto it = ctx->prof_index.find(name);
     if (it == ctx->prof_index.end()) {
         ctx->prof.emplace_back();
@@ -385,6 +385,7 @@ static atr_status tree_alloc(atr_context* ctx, uint32_t n, atr_tree** out)
     atr_tree* t = new atr_tree();
     t->ctx = ctx;
     t->n = n;
+    t->first_leaf = n - 1;
     const size_t m = 2ull * n - 1;
     atr_status s = ATR_OK;
     if (s == ATR_OK) s = dalloc_t(ctx, m, &t->parent);
@@ -447,8 +448,8 @@ atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out)
                        blocks, slots, n - 1, nullptr, 0);
     bool in_alt = false;
     if (s == ATR_OK)
-        s = run_sort(ctx, kb * 8, keys[0], vals[0], n, keys[1], vals[1], true, hist, blocks,
-                     &in_alt);
+        s = run_sort(ctx, kb * 8, passes, keys[0], vals[0], n, keys[1], vals[1], true, hist,
+                     blocks, nullptr, &in_alt);
     if (s == ATR_OK)
         s = run_build(ctx, morton_bits, keys[in_alt ? 1 : 0], vals[in_alt ? 1 : 0], n,
                       scene->verts, t, slots);
@@ -853,8 +854,8 @@ atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t
     const bool identity = d_values == nullptr;
     if (identity) ATR_TRY(dalloc_t(ctx, n, &vtmp));
     bool in_alt = false;
-    atr_status s = run_sort(ctx, key_bits, d_keys, identity ? vtmp : d_values, n, kalt, valt,
-                            identity, nullptr, 0, &in_alt);
+    atr_status s = run_sort(ctx, key_bits, key_bits / 8, d_keys, identity ? vtmp : d_values, n,
+                            kalt, valt, identity, nullptr, 0, nullptr, &in_alt);
     if (s == ATR_OK && in_alt) {
         hipError_t e = hipMemcpyAsync(d_keys, kalt, (size_t)n * key_bits / 8,
                                       hipMemcpyDeviceToDevice, ctx->stream);
@@ -880,6 +881,250 @@ atr_status atr_refit(atr_tree* t, atr_scene* scene)
     atr_status s = run_refit(t->ctx, t, scene->verts, counters);
     dfree(t->ctx, counters);
     return s;
+}
+
+// --------------------------------------------------------- batched builds
+static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_device,
+                               const uint32_t* tri_offsets, uint32_t meshes,
+                               const float* bboxes, atr_batch** out)
+{
+    if (!ctx || !vertices || !tri_offsets || !out)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (meshes == 0) return fail(ATR_ERR_INVALID_ARGUMENT, "a batch needs at least one mesh");
+    if (tri_offsets[0] != 0) return fail(ATR_ERR_INVALID_ARGUMENT, "tri_offsets[0] must be 0");
+    for (uint32_t m = 0; m < meshes; ++m)
+        if (tri_offsets[m + 1] < tri_offsets[m] + 2)
+            return fail(ATR_ERR_INVALID_ARGUMENT,
+                        "mesh " + std::to_string(m) + " has fewer than 2 triangles");
+    const uint32_t n = tri_offsets[meshes];
+    if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 triangles per batch");
+    if (!on_device) {
+        for (size_t i = 0; i < 3ull * n; ++i)
+            for (int a = 0; a < 3; ++a)
+                if (!std::isfinite(vertices[i * 4 + a]))
+                    return fail(ATR_ERR_INVALID_ARGUMENT,
+                                "non-finite vertex coordinate at vertex " + std::to_string(i));
+    }
+    atr_batch* b = new atr_batch();
+    b->ctx = ctx;
+    b->meshes = meshes;
+    b->n = n;
+    b->h_seg.assign(tri_offsets, tri_offsets + meshes + 1);
+    b->passes = forest_passes(meshes);
+    b->hist_blocks = std::max(encode_blocks(n), 1u);
+    atr_status s = ATR_OK;
+    if (s == ATR_OK) s = dalloc_t(ctx, meshes + 1, &b->d_seg);
+    if (s == ATR_OK) s = dalloc_t(ctx, 3ull * n, &b->verts);
+    if (s == ATR_OK) s = dalloc_t(ctx, 6ull * meshes, &b->d_bbox);
+    if (s == ATR_OK) s = dalloc_t(ctx, 6ull * meshes, &b->bbox_acc);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->keys[0]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->keys[1]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->vals[0]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->vals[1]);
+    if (s == ATR_OK) s = dalloc_t(ctx, (size_t)b->hist_blocks * b->passes * 256, &b->hist);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->slots);
+    if (s == ATR_OK) s = dalloc_t(ctx, (size_t)b->passes * 256, &b->sort_offsets);
+    if (s == ATR_OK) s = dalloc_t(ctx, sort_status_words(n, b->passes), &b->sort_status);
+    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &b->tris);
+    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &b->nsah);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->counters);
+    if (s == ATR_OK) s = dalloc_t(ctx, meshes, &b->d_sah);
+    if (s == ATR_OK) {
+        atr_tree* f = new atr_tree();
+        f->ctx = ctx;
+        f->n = n;
+        f->first_leaf = n;
+        b->forest = f;
+        const size_t m = 2ull * n;
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->parent);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->left);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->right);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->data);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->bbmin);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->bbmax);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->area);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &f->span);
+    }
+    if (s == ATR_OK) {
+        hipError_t e = hipMemcpyAsync(b->d_seg, tri_offsets, (meshes + 1ull) * 4,
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->verts, vertices, 3ull * n * sizeof(float4),
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               ctx->stream);
+        if (e == hipSuccess && bboxes) {
+            b->bbox_given = true;
+            e = hipMemcpyAsync(b->d_bbox, bboxes, 6ull * meshes * 4, hipMemcpyHostToDevice,
+                               ctx->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, std::string("batch upload: ") +
+                                                          hipGetErrorString(e));
+    }
+    if (s != ATR_OK) {
+        atr_batch_destroy(b);
+        return s;
+    }
+    *out = b;
+    return ATR_OK;
+}
+
+atr_status atr_batch_create(atr_context* ctx, const float* vertices, const uint32_t* tri_offsets,
+                            uint32_t num_meshes, const float* bboxes, atr_batch** out)
+{
+    return batch_create(ctx, vertices, false, tri_offsets, num_meshes, bboxes, out);
+}
+
+atr_status atr_batch_create_device(atr_context* ctx, const float* d_vertices,
+                                   const uint32_t* tri_offsets, uint32_t num_meshes,
+                                   const float* bboxes, atr_batch** out)
+{
+    return batch_create(ctx, d_vertices, true, tri_offsets, num_meshes, bboxes, out);
+}
+
+atr_status atr_batch_vertices(atr_batch* b, float** d_vertices)
+{
+    if (!b || !d_vertices) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *d_vertices = (float*)b->verts;
+    return ATR_OK;
+}
+
+// The whole batched build as one stream-ordered sequence of kernels over
+// persistent buffers: no allocation, no memset node, no host synchronisation,
+// so it can be captured into a graph.
+static atr_status batch_enqueue(atr_batch* b, int treelet, int iterations, float ci, float ct)
+{
+    atr_context* ctx = b->ctx;
+    const uint32_t n = b->n;
+    if (!b->bbox_given)
+        ATR_TRY(run_bbox_forest(ctx, b->verts, b->d_seg, b->meshes, n, b->bbox_acc, b->d_bbox));
+    ATR_TRY(run_encode_forest(ctx, b->verts, b->d_seg, b->meshes, n, b->d_bbox, b->keys[0],
+                              b->hist, b->slots, n));
+    SortWork sw{b->sort_offsets, b->sort_status};
+    bool in_alt = false;
+    ATR_TRY(run_sort(ctx, 64, b->passes, b->keys[0], b->vals[0], n, b->keys[1], b->vals[1], true,
+                     b->hist, b->hist_blocks, &sw, &in_alt));
+    const int k = in_alt ? 1 : 0;
+    ATR_TRY(run_build_forest(ctx, b->keys[k], b->vals[k], b->verts, b->forest, b->d_seg,
+                             b->slots));
+    int gamma = treelet;
+    for (int it = 0; it < iterations; ++it) {
+        ATR_TRY(run_atrbvh_pass(ctx, b->forest, treelet, gamma, ci, ct, b->tris, b->nsah,
+                                b->counters));
+        gamma *= 2;
+    }
+    return ATR_OK;
+}
+
+static void batch_drop_graph(atr_batch* b)
+{
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float ci, float ct,
+                           int use_graph)
+{
+    if (!b) return fail(ATR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (treelet_size < 3 || treelet_size > 32)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
+    if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    atr_context* ctx = b->ctx;
+    if (!use_graph) {
+        ATR_TRY(batch_enqueue(b, treelet_size, iterations, ci, ct));
+        b->built = true;
+        return ATR_OK;
+    }
+    const bool same = b->graph_exec && b->g_treelet == treelet_size &&
+                      b->g_iterations == iterations && b->g_ci == ci && b->g_ct == ct;
+    if (!same) {
+        batch_drop_graph(b);
+        ATR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        atr_status s = batch_enqueue(b, treelet_size, iterations, ci, ct);
+        ctx->capturing = false;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (s != ATR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return s;
+        }
+        if (e != hipSuccess)
+            return fail(ATR_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        b->graph = g;
+        e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            batch_drop_graph(b);
+            return fail(ATR_ERR_DEVICE, std::string("graph instantiate: ") +
+                                            hipGetErrorString(e));
+        }
+        b->g_treelet = treelet_size;
+        b->g_iterations = iterations;
+        b->g_ci = ci;
+        b->g_ct = ct;
+    }
+    ATR_HIP(hipGraphLaunch(b->graph_exec, ctx->stream));
+    b->built = true;
+    return ATR_OK;
+}
+
+atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out)
+{
+    if (!b || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
+    atr_context* ctx = b->ctx;
+    ATR_TRY(run_forest_sah(ctx, b->forest, b->d_seg, b->meshes, ci, ct, b->d_sah));
+    ATR_HIP(hipMemcpyAsync(out, b->d_sah, (size_t)b->meshes * 4, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ATR_HIP(hipStreamSynchronize(ctx->stream));
+    return ATR_OK;
+}
+
+atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out)
+{
+    if (!b || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (mesh >= b->meshes) return fail(ATR_ERR_INVALID_ARGUMENT, "mesh index out of range");
+    if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
+    const uint32_t s0 = b->h_seg[mesh];
+    const uint32_t nm = b->h_seg[mesh + 1] - s0;
+    atr_tree* t = nullptr;
+    ATR_TRY(tree_alloc(b->ctx, nm, &t));
+    atr_status s = run_forest_extract(b->ctx, b->forest, s0, nm, t);
+    if (s != ATR_OK) {
+        atr_tree_destroy(t);
+        return s;
+    }
+    *out = t;
+    return ATR_OK;
+}
+
+atr_status atr_batch_destroy(atr_batch* b)
+{
+    if (!b) return ATR_OK;
+    atr_context* ctx = b->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    batch_drop_graph(b);
+    if (b->forest) atr_tree_destroy(b->forest);
+    dfree(ctx, b->d_seg);
+    dfree(ctx, b->verts);
+    dfree(ctx, b->d_bbox);
+    dfree(ctx, b->bbox_acc);
+    dfree(ctx, b->keys[0]);
+    dfree(ctx, b->keys[1]);
+    dfree(ctx, b->vals[0]);
+    dfree(ctx, b->vals[1]);
+    dfree(ctx, b->hist);
+    dfree(ctx, b->slots);
+    dfree(ctx, b->sort_offsets);
+    dfree(ctx, b->sort_status);
+    dfree(ctx, b->tris);
+    dfree(ctx, b->nsah);
+    dfree(ctx, b->counters);
+    dfree(ctx, b->d_sah);
+    delete b;
+    return ATR_OK;
 }
 
 }  // extern "C"

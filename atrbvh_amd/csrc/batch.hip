@@ -1,0 +1,88 @@
+// Batched builds: per-mesh views of a forest built in one launch sequence.
+//
+// A forest holds M independent meshes in one mesh-major triangle array
+// (seg[m] = first triangle of mesh m).  Internal node slot g < n belongs to
+// the mesh whose sorted positions contain g; leaves start at first_leaf = n.
+// Each mesh's tree, renumbered to its own BVHRT layout (internal 0..Nm-2,
+// leaves Nm-1..2Nm-2, root 0), is exactly what a build of that mesh alone
+// gives.
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace atr {
+
+namespace {
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Renumbers one mesh of a forest into a standalone tree.
+__global__ __launch_bounds__(256) void k_forest_extract(
+    uint32_t s, uint32_t nm, int first_leaf, const int32_t* __restrict__ P,
+    const int32_t* __restrict__ L, const int32_t* __restrict__ R, const int32_t* __restrict__ D,
+    const float4* __restrict__ bmin, const float4* __restrict__ bmax,
+    const float* __restrict__ A, const int2* __restrict__ SP, int32_t* parent, int32_t* left,
+    int32_t* right, int32_t* data, float4* bbmin, float4* bbmax, float* area, int2* span)
+{
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= 2u * nm - 1u) return;
+    const int last = (int)nm - 1;
+    const int g = (int)x < last ? (int)(s + x) : first_leaf + (int)s + ((int)x - last);
+    auto conv = [&](int y) -> int32_t {
+        if (y < 0) return -1;
+        return y < first_leaf ? y - (int)s : last + (y - first_leaf - (int)s);
+    };
+    parent[x] = conv(P[g]);
+    left[x] = conv(L[g]);
+    right[x] = conv(R[g]);
+    const int d = D[g];
+    data[x] = d >= 0 ? d - (int)s : -1;
+    bbmin[x] = bmin[g];
+    bbmax[x] = bmax[g];
+    area[x] = A[g];
+    const int2 sp = SP[g];
+    span[x] = make_int2(sp.x - (int)s, sp.y - (int)s);
+}
+
+// SoABVHTree::SAH (BVHRT-Core/src/SoABVHTree.cpp:50-74) of every mesh, one
+// thread per mesh, summed serially in the reference's order: root, internal
+// nodes 1..Nm-2, then the leaves.
+__global__ __launch_bounds__(64) void k_forest_sah(const uint32_t* __restrict__ seg,
+                                                   uint32_t meshes, int first_leaf,
+                                                   const float* __restrict__ area, float ci,
+                                                   float ct, float* out)
+{
+    const uint32_t m = blockIdx.x * 64u + threadIdx.x;
+    if (m >= meshes) return;
+    const uint32_t s = seg[m], e = seg[m + 1];
+    const float total = area[s];
+    float si = total;
+    for (uint32_t g = s + 1; g + 1 < e; ++g) si += area[g];
+    float sl = 0.0f;
+    for (uint32_t p = s; p < e; ++p) sl += area[first_leaf + p];
+    out[m] = (ci * si + ct * sl) / total;
+}
+
+}  // namespace
+
+atr_status run_forest_extract(atr_context* ctx, const atr_tree* f, uint32_t s, uint32_t nm,
+                              atr_tree* t)
+{
+    return launch(ctx, "forest_extract", (2.0 * nm - 1.0) * 104.0, [&] {
+        k_forest_extract<<<div_up(2ull * nm - 1, 256), 256, 0, ctx->stream>>>(
+            s, nm, (int)f->first_leaf, f->parent, f->left, f->right, f->data, f->bbmin, f->bbmax,
+            f->area, f->span, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
+            t->span);
+    });
+}
+
+atr_status run_forest_sah(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,
+                          uint32_t meshes, float ci, float ct, float* d_out)
+{
+    return launch(ctx, "forest_sah", (double)f->n * 8.0, [&] {
+        k_forest_sah<<<div_up(meshes, 64), 64, 0, ctx->stream>>>(d_seg, meshes,
+                                                                 (int)f->first_leaf, f->area, ci,
+                                                                 ct, d_out);
+    });
+}
+
+}  // namespace atr

@@ -52,6 +52,9 @@ struct atr_context {
     std::vector<atr::ProfileRecord> prof;
     std::map<std::string, int> prof_index;
     std::vector<hipEvent_t> event_pool;
+    // true while the stream is being captured into a HIP graph (no profiling
+    // events are recorded then)
+    bool capturing = false;
     // device status word (bounded spin-wait overflow etc.)
     uint32_t* d_status = nullptr;
     // pinned host staging for small readbacks, handed out in fixed slots
@@ -71,7 +74,12 @@ struct atr_scene {
 struct atr_tree {
     atr_context* ctx = nullptr;
     atr_scene* scene = nullptr;  // scene the tree was built from (may be NULL)
-    uint32_t n = 0;
+    uint32_t n = 0;  // leaves (triangles)
+    // Index of the first leaf node.  A single tree has internal nodes
+    // 0..n-2 and first_leaf = n-1.  A forest of independent meshes (batched
+    // builds) gives internal node slots 0..n-1 (the slot at each mesh's last
+    // sorted position is unused) and first_leaf = n.
+    uint32_t first_leaf = 0;
     int32_t* parent = nullptr;
     int32_t* left = nullptr;
     int32_t* right = nullptr;
@@ -115,6 +123,39 @@ struct atr_collapsed {
     hipEvent_t summary_event = nullptr;
 };
 
+// Many independent meshes built as one forest (batched builds).
+struct atr_batch {
+    atr_context* ctx = nullptr;
+    uint32_t meshes = 0;
+    uint32_t n = 0;                // total triangles
+    std::vector<uint32_t> h_seg;   // meshes + 1 prefix offsets
+    uint32_t* d_seg = nullptr;
+    float4* verts = nullptr;       // 3n, owned
+    float* d_bbox = nullptr;       // 6 per mesh
+    bool bbox_given = false;
+    uint32_t* bbox_acc = nullptr;  // 6 per mesh
+    atr_tree* forest = nullptr;    // first_leaf = n, 2n node slots
+    // persistent workspace (every build, eager or graph, uses these)
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;
+    uint32_t hist_blocks = 0;
+    int passes = 0;
+    uint64_t* slots = nullptr;
+    uint32_t* sort_offsets = nullptr;
+    uint32_t* sort_status = nullptr;
+    int32_t* tris = nullptr;
+    float* nsah = nullptr;
+    uint32_t* counters = nullptr;
+    float* d_sah = nullptr;        // per-mesh SAH
+    bool built = false;
+    // captured build sequence
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int g_treelet = -1, g_iterations = -1;
+    float g_ci = 0.0f, g_ct = 0.0f;
+};
+
 namespace atr {
 
 // ------------------------------------------------------------- allocation
@@ -152,15 +193,39 @@ atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const f
                       int bits, void* keys, uint32_t* idx, uint32_t* block_hist,
                       uint32_t hist_blocks, uint64_t* fill64, uint32_t fill64_count,
                       uint32_t* zero32, uint32_t zero32_count);
-atr_status run_sort(atr_context* ctx, int key_bits, void* keys, uint32_t* vals, uint32_t n,
-                    void* keys_alt, uint32_t* vals_alt, bool vals_identity,
-                    const uint32_t* block_hist, uint32_t hist_blocks, bool* result_in_alt);
+// Optional persistent sort workspace (digit offsets: passes*256 words;
+// look-back status: sort_status_words(n, passes) words).
+struct SortWork {
+    uint32_t* offsets;
+    uint32_t* status;
+};
+size_t sort_status_words(uint32_t n, int passes);
+atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint32_t* vals,
+                    uint32_t n, void* keys_alt, uint32_t* vals_alt, bool vals_identity,
+                    const uint32_t* block_hist, uint32_t hist_blocks, const SortWork* work,
+                    bool* result_in_alt);
+// Forests (batched builds): per-mesh boxes, mesh-tagged 30-bit keys, build.
+int forest_passes(uint32_t meshes);
+atr_status run_bbox_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
+                           uint32_t meshes, uint32_t n, uint32_t* acc, float* d_bbox);
+atr_status run_encode_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
+                             uint32_t meshes, uint32_t n, const float* d_bbox, uint64_t* keys,
+                             uint32_t* block_hist, uint64_t* fill64, uint32_t fill64_count);
+atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32_t* idx,
+                            const float4* verts, atr_tree* t, const uint32_t* d_seg,
+                            uint64_t* slots);
 uint32_t encode_blocks(uint32_t n);
 atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
                      uint32_t n, const float4* verts, atr_tree* t, uint64_t* slots);
 atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_t* counters);
 // Host: spans of a tree given as host arrays (post-order over the topology).
 void host_spans(uint32_t n, const int32_t* left, const int32_t* right, int2* span);
+
+// Batched builds (batch.hip)
+atr_status run_forest_extract(atr_context* ctx, const atr_tree* f, uint32_t s, uint32_t nm,
+                              atr_tree* t);
+atr_status run_forest_sah(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,
+                          uint32_t meshes, float ci, float ct, float* d_out);
 
 // Treelets (treelet.hip)
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,

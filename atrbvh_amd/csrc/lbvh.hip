@@ -162,6 +162,155 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(
     }
 }
 
+// ---------------------------------------------------------------- forests
+// Batched builds keep many meshes in one mesh-major triangle array; seg[m] is
+// the first triangle of mesh m and seg[M] the total.
+__device__ __forceinline__ uint32_t mesh_of(const uint32_t* seg, uint32_t meshes, uint32_t i)
+{
+    uint32_t lo = 0, hi = meshes;  // seg[lo] <= i < seg[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_fill_bbox_acc(uint32_t* acc, uint32_t meshes)
+{
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < 6u * meshes; k += 256u * gridDim.x)
+        acc[k] = (k % 6u) < 3u ? f2o(FLT_MAX) : f2o(FLT_MIN);
+}
+
+// Per-mesh scene boxes with SceneConverter's rule (SceneConverter.cpp:28-64).
+// Each wave walks a contiguous run of triangles, each lane keeps a running box
+// for its current mesh and flushes it with orderable-uint atomics when the
+// mesh changes and at the end of the run (min/max are exact, so the order of
+// the atomics does not matter).
+constexpr int kBoxRun = 64 * 32;  // triangles per wave run
+__global__ __launch_bounds__(256) void k_bbox_forest(const float4* __restrict__ v,
+                                                     const uint32_t* __restrict__ seg,
+                                                     uint32_t meshes, uint32_t n, uint32_t* acc)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t begin = wave * (uint32_t)kBoxRun;
+    if (begin >= n) return;
+    uint32_t m = mesh_of(seg, meshes, begin + lane < n ? begin + lane : begin);
+    uint32_t next = seg[m + 1];
+    float mn0 = FLT_MAX, mn1 = FLT_MAX, mn2 = FLT_MAX;
+    float mx0 = FLT_MIN, mx1 = FLT_MIN, mx2 = FLT_MIN;
+    bool any = false;
+    for (uint32_t j = 0; j < (uint32_t)kBoxRun / 64u; ++j) {
+        const uint32_t i = begin + j * 64u + lane;
+        if (i >= n) break;
+        if (i >= next) {
+            if (any) {
+                atomicMin(&acc[6 * m + 0], f2o(mn0));
+                atomicMin(&acc[6 * m + 1], f2o(mn1));
+                atomicMin(&acc[6 * m + 2], f2o(mn2));
+                atomicMax(&acc[6 * m + 3], f2o(mx0));
+                atomicMax(&acc[6 * m + 4], f2o(mx1));
+                atomicMax(&acc[6 * m + 5], f2o(mx2));
+            }
+            while (i >= next) next = seg[++m + 1];
+            mn0 = mn1 = mn2 = FLT_MAX;
+            mx0 = mx1 = mx2 = FLT_MIN;
+            any = false;
+        }
+        const float4 a = v[3ull * i];
+        const float4 b = v[3ull * i + 1];
+        const float4 c = v[3ull * i + 2];
+        mn0 = fminf(mn0, fminf(fminf(a.x, b.x), c.x));
+        mn1 = fminf(mn1, fminf(fminf(a.y, b.y), c.y));
+        mn2 = fminf(mn2, fminf(fminf(a.z, b.z), c.z));
+        mx0 = fmaxf(mx0, fmaxf(fmaxf(a.x, b.x), c.x));
+        mx1 = fmaxf(mx1, fmaxf(fmaxf(a.y, b.y), c.y));
+        mx2 = fmaxf(mx2, fmaxf(fmaxf(a.z, b.z), c.z));
+        any = true;
+    }
+    // common case: the whole wave ends in one mesh -> one set of atomics
+    const uint32_t m0 = __shfl(m, 0, 64);
+    const bool uniform = __ballot(m != m0 && any) == 0ull;
+    if (uniform) {
+        for (int o = 32; o > 0; o >>= 1) {
+            mn0 = fminf(mn0, __shfl_xor(mn0, o, 64));
+            mn1 = fminf(mn1, __shfl_xor(mn1, o, 64));
+            mn2 = fminf(mn2, __shfl_xor(mn2, o, 64));
+            mx0 = fmaxf(mx0, __shfl_xor(mx0, o, 64));
+            mx1 = fmaxf(mx1, __shfl_xor(mx1, o, 64));
+            mx2 = fmaxf(mx2, __shfl_xor(mx2, o, 64));
+        }
+        any = lane == 0u;
+        m = m0;
+    }
+    if (any) {
+        atomicMin(&acc[6 * m + 0], f2o(mn0));
+        atomicMin(&acc[6 * m + 1], f2o(mn1));
+        atomicMin(&acc[6 * m + 2], f2o(mn2));
+        atomicMax(&acc[6 * m + 3], f2o(mx0));
+        atomicMax(&acc[6 * m + 4], f2o(mx1));
+        atomicMax(&acc[6 * m + 5], f2o(mx2));
+    }
+}
+
+__global__ void k_bbox_forest_final(const uint32_t* acc, uint32_t meshes, float* bbox)
+{
+    for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < 6u * meshes; k += 256u * gridDim.x)
+        bbox[k] = o2f(acc[k]);
+}
+
+// Forest keys: mesh << 30 | 30-bit Morton code of the triangle in its own
+// mesh's box, and the per-block digit histograms of `passes` 8-bit digits.
+__global__ __launch_bounds__(kEncBlock) void k_encode_forest(
+    const float4* __restrict__ v, const uint32_t* __restrict__ seg, uint32_t meshes, uint32_t n,
+    const float* __restrict__ bbox, int passes, uint64_t* __restrict__ keys,
+    uint32_t* __restrict__ block_hist, uint64_t* __restrict__ fill64, uint32_t fill64_count)
+{
+    __shared__ uint32_t hist[kEncBlock / 64][8 * 256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    for (int k = tid; k < (kEncBlock / 64) * passes * 256; k += kEncBlock) {
+        const int w = k / (passes * 256);
+        hist[w][k - w * passes * 256] = 0;
+    }
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)kEncTile;
+    const uint32_t first = base + (uint32_t)tid;
+    uint32_t m = first < n ? mesh_of(seg, meshes, first) : 0u;
+    uint32_t next = seg[m + 1];
+    for (int j = 0; j < kEncItems; ++j) {
+        const uint32_t i = base + (uint32_t)j * kEncBlock + (uint32_t)tid;
+        if (i >= n) break;
+        while (i >= next) next = seg[++m + 1];
+        const float* bb = bbox + 6ull * m;
+        const float4 a = v[3ull * i];
+        const float4 b = v[3ull * i + 1];
+        const float4 c = v[3ull * i + 2];
+        float cx = (fmaxf(fmaxf(a.x, b.x), c.x) + fminf(fminf(a.x, b.x), c.x)) * 0.5f;
+        float cy = (fmaxf(fmaxf(a.y, b.y), c.y) + fminf(fminf(a.y, b.y), c.y)) * 0.5f;
+        float cz = (fmaxf(fmaxf(a.z, b.z), c.z) + fminf(fminf(a.z, b.z), c.z)) * 0.5f;
+        const uint32_t code = MortonTraits<uint32_t>::code(
+            (cx - bb[0]) / (bb[3] - bb[0]), (cy - bb[1]) / (bb[4] - bb[1]),
+            (cz - bb[2]) / (bb[5] - bb[2]));
+        const uint64_t key = ((uint64_t)m << 30) | code;
+        keys[i] = key;
+        for (int p = 0; p < passes; ++p)
+            atomicAdd(&hist[wave][p * 256 + (uint32_t)((key >> (8 * p)) & 255u)], 1u);
+    }
+    const uint32_t gsize = gridDim.x * kEncBlock;
+    const uint32_t gtid = blockIdx.x * kEncBlock + tid;
+    for (uint32_t k = gtid; k < fill64_count; k += gsize) fill64[k] = ~0ull;
+    __syncthreads();
+    uint32_t* out = block_hist + (size_t)blockIdx.x * passes * 256;
+    for (int e = tid; e < passes * 256; e += kEncBlock) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < kEncBlock / 64; ++w) s += hist[w][e];
+        out[e] = s;
+    }
+}
+
 // Global digit offsets for every pass: offsets[p][d] = number of keys whose
 // digit p is smaller than d.  One block per pass, one thread per digit.
 __global__ __launch_bounds__(256) void k_digit_offsets(const uint32_t* __restrict__ block_hist,
@@ -365,20 +514,51 @@ __device__ __forceinline__ int delta_at(const uint64_t* k, int i)
 // A split whose node lies entirely inside this block's 256 leaves only ever
 // sees this block's threads, so its exchange (and the box handed over) lives
 // in LDS with workgroup-scope ordering.  Only splits whose node crosses a
-// block boundary use the global slot with agent-scope acquire/release (which
-// on gfx950 writes back / invalidates the XCD's L2).  A split s is block-local
-// iff the nearest smaller delta on both sides lies inside the block, i.e. the
-// prefix minimum of delta over [base-1, s-1] and the suffix minimum over
-// [s+1, base+255] are both below delta(s).
+// block boundary use the global slot, with the cross-XCD hand-over of
+// device.hpp (coherent stores, relaxed agent-scope exchange, coherent loads).
+// A split s is block-local iff the nearest smaller delta on both sides lies
+// inside the block, i.e. the prefix minimum of delta over [base-1, s-1] and
+// the suffix minimum over [s+1, base+255] are both below delta(s).
+// Sorted-key views for the build.  delta(i) is the split strength between
+// sorted positions i and i+1; -1 marks a wall that no node spans (the ends of
+// the array, and in a forest the boundary between two meshes).
 template <typename K>
-__device__ __forceinline__ int delta_or_end(const K* k, int i, int last)
-{
-    return (i < 0 || i >= last) ? -1 : delta_at(k, i);
-}
+struct SingleKeys {
+    const K* k;
+    int last;  // n - 1
+    __device__ __forceinline__ int delta(int i) const
+    {
+        return (i < 0 || i >= last) ? -1 : delta_at(k, i);
+    }
+};
 
-template <typename K>
-__global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
-                                               const uint32_t* __restrict__ idx, uint32_t n,
+// Forest of independent meshes sorted together: key = mesh << 30 | 30-bit
+// Morton code, mesh-major.  Within a mesh delta equals the single-tree
+// 30-bit delta with positions counted from the mesh's first position, so each
+// mesh gets exactly the tree a build of that mesh alone gives.
+struct ForestKeys {
+    const uint64_t* k;
+    int last;
+    const uint32_t* seg_start;  // first sorted position of every mesh
+    __device__ __forceinline__ int delta(int i) const
+    {
+        if (i < 0 || i >= last) return -1;
+        const uint64_t a = k[i], b = k[i + 1];
+        if ((a ^ b) >> 30) return -1;
+        const uint32_t x = (uint32_t)a ^ (uint32_t)b;
+        if (x) return __clz(x);
+        const int s = (int)seg_start[a >> 30];
+        return 32 + __clz((uint32_t)((i - s) ^ (i + 1 - s)));
+    }
+};
+
+// Node numbering: the node covering sorted range [lo, hi] is stored at hi when
+// it is a left child, at lo when it is a right child, and a root (both ends at
+// walls) at lo -- position 0 for a single tree, the mesh's first position in a
+// forest.  Leaf of sorted position i is node first_leaf + i.
+template <typename KV>
+__global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restrict__ idx,
+                                               uint32_t n, int first_leaf,
                                                const float4* __restrict__ v, int32_t* parent,
                                                int32_t* left, int32_t* right, int32_t* data,
                                                float4* bbmin, float4* bbmax, float* area,
@@ -393,11 +573,10 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int last = (int)n - 1;
     const int base = (int)(blockIdx.x * 256u);
     // delta at positions base-1 .. base+255 (index t <-> position base-1+t)
-    s_d[tid] = delta_or_end(keys, base - 1 + tid, last);
-    if (tid == 0) s_d[256] = delta_or_end(keys, base + 255, last);
+    s_d[tid] = keys.delta(base - 1 + tid);
+    if (tid == 0) s_d[256] = keys.delta(base + 255);
     s_slot[tid] = ~0ull;
     __syncthreads();
     // inclusive prefix minimum of s_d[0..tid] and suffix minimum of s_d[tid+1..256]
@@ -442,7 +621,7 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
                             fminf(fminf(a.z, b.z), c.z), 0.0f);
     float4 mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
                             fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
-    int node = last + (int)i;
+    int node = first_leaf + (int)i;
     data[node] = (int32_t)tri;
     left[node] = -1;
     right[node] = -1;
@@ -453,10 +632,10 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
     span[node] = make_int2((int)i, (int)i);
 
     int lo = (int)i, hi = (int)i;
-    int dl = lo > 0 ? delta_at(keys, lo - 1) : -1;
-    int dr = hi < last ? delta_at(keys, hi) : -1;
+    int dl = keys.delta(lo - 1);
+    int dr = keys.delta(hi);
     while (true) {
-        if (lo == 0 && hi == last) {
+        if (dl < 0 && dr < 0) {
             parent[node] = -1;
             break;
         }
@@ -492,14 +671,12 @@ __global__ __launch_bounds__(256) void k_build(const K* __restrict__ keys,
         const int rchild = is_left ? sib : node;
         if (is_left) {
             hi = sfar;
-            dr = hi < last ? delta_at(keys, hi) : -1;
+            dr = keys.delta(hi);
         } else {
             lo = sfar;
-            dl = lo > 0 ? delta_at(keys, lo - 1) : -1;
+            dl = keys.delta(lo - 1);
         }
-        int p;
-        if (lo == 0 && hi == last) p = 0;
-        else p = dr > dl ? hi : lo;
+        const int p = (dl < 0 && dr < 0) ? lo : (dr > dl ? hi : lo);
         left[p] = lchild;
         right[p] = rchild;
         data[p] = -1;
@@ -605,22 +782,33 @@ atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const f
     });
 }
 
-// Stable LSD sort, 8-bit digits over all key bits.  Results alternate between
-// (keys, vals) and (keys_alt, vals_alt); *result_in_alt says where they end.
-// block_hist may come from the encode kernel; when NULL it is computed here.
-atr_status run_sort(atr_context* ctx, int key_bits, void* keys, uint32_t* vals, uint32_t n,
-                    void* keys_alt, uint32_t* vals_alt, bool vals_identity,
-                    const uint32_t* block_hist, uint32_t hist_blocks, bool* result_in_alt)
+size_t sort_status_words(uint32_t n, int passes)
 {
-    const int passes = key_bits == 64 ? 8 : 4;
+    return (size_t)passes * (std::max(div_up(n, kSortTile), 1u) * 256u + 64u);
+}
+
+// Stable LSD sort over `passes` 8-bit digits (the low 8*passes key bits).
+// Results alternate between (keys, vals) and (keys_alt, vals_alt);
+// *result_in_alt says where they end.  block_hist may come from the encode
+// kernel; when NULL it is computed here.  work (optional) supplies the digit
+// offsets (passes*256 words) and look-back status (sort_status_words) so that
+// nothing is allocated (graph capture); otherwise they are allocated here.
+atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint32_t* vals,
+                    uint32_t n, void* keys_alt, uint32_t* vals_alt, bool vals_identity,
+                    const uint32_t* block_hist, uint32_t hist_blocks, const SortWork* work,
+                    bool* result_in_alt)
+{
     const int kbytes = key_bits == 64 ? 8 : 4;
+    if (passes < 1 || passes > kbytes) return fail(ATR_ERR_INTERNAL, "bad sort pass count");
     const uint32_t tiles = std::max(div_up(n, kSortTile), 1u);
-    uint32_t* offsets = nullptr;
-    uint32_t* status = nullptr;
+    uint32_t* offsets = work ? work->offsets : nullptr;
+    uint32_t* status = work ? work->status : nullptr;
     uint32_t* own_hist = nullptr;
-    ATR_TRY(dalloc_t(ctx, (size_t)passes * 256, &offsets));
-    const size_t status_words = (size_t)passes * (tiles * 256u + 64u);
-    ATR_TRY(dalloc_t(ctx, status_words, &status));
+    const size_t status_words = sort_status_words(n, passes);
+    if (!work) {
+        ATR_TRY(dalloc_t(ctx, (size_t)passes * 256, &offsets));
+        ATR_TRY(dalloc_t(ctx, status_words, &status));
+    }
     if (!block_hist) {
         hist_blocks = std::max(div_up(n, kEncTile), 1u);
         ATR_TRY(dalloc_t(ctx, (size_t)hist_blocks * passes * 256, &own_hist));
@@ -670,11 +858,51 @@ atr_status run_sort(atr_context* ctx, int key_bits, void* keys, uint32_t* vals, 
         cur = 1 - cur;
     }
     const bool in_alt = cur == 1;
-    dfree(ctx, offsets);
-    dfree(ctx, status);
+    if (!work) {
+        dfree(ctx, offsets);
+        dfree(ctx, status);
+    }
     if (own_hist) dfree(ctx, own_hist);
     *result_in_alt = in_alt;
     return ATR_OK;
+}
+
+// Forest stages (batched builds).
+atr_status run_bbox_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
+                           uint32_t meshes, uint32_t n, uint32_t* acc, float* d_bbox)
+{
+    const uint32_t g6 = std::max(std::min<uint32_t>(div_up(6ull * meshes, 256), 1024u), 1u);
+    ATR_TRY(launch(ctx, "bbox_forest_init", meshes * 24.0, [&] {
+        k_fill_bbox_acc<<<g6, 256, 0, ctx->stream>>>(acc, meshes);
+    }));
+    const uint32_t waves = div_up(n, kBoxRun);
+    ATR_TRY(launch(ctx, "bbox_forest", (double)n * 48.0, [&] {
+        k_bbox_forest<<<std::max(div_up(waves, 4), 1u), 256, 0, ctx->stream>>>(verts, d_seg,
+                                                                                 meshes, n, acc);
+    }));
+    return launch(ctx, "bbox_forest_final", meshes * 48.0, [&] {
+        k_bbox_forest_final<<<g6, 256, 0, ctx->stream>>>(acc, meshes, d_bbox);
+    });
+}
+
+int forest_passes(uint32_t meshes)
+{
+    int mbits = 0;
+    while (mbits < 34 && (1ull << mbits) < meshes) ++mbits;
+    return (30 + mbits + 7) / 8;
+}
+
+atr_status run_encode_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
+                             uint32_t meshes, uint32_t n, const float* d_bbox, uint64_t* keys,
+                             uint32_t* block_hist, uint64_t* fill64, uint32_t fill64_count)
+{
+    const int passes = forest_passes(meshes);
+    const uint32_t blocks = std::max(encode_blocks(n), 1u);
+    return launch(ctx, "encode_forest", (double)n * 56.0, [&] {
+        k_encode_forest<<<blocks, kEncBlock, 0, ctx->stream>>>(verts, d_seg, meshes, n, d_bbox,
+                                                               passes, keys, block_hist, fill64,
+                                                               fill64_count);
+    });
 }
 
 atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
@@ -682,18 +910,33 @@ atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_
 {
     const uint32_t grid = div_up(n, 256);
     const double bytes = (double)n * 176.0;
+    const int fl = (int)t->first_leaf;
     if (bits == 63) {
+        SingleKeys<uint64_t> kv{(const uint64_t*)keys, (int)n - 1};
         return launch(ctx, "build_refit63", bytes, [&] {
-            k_build<uint64_t><<<grid, 256, 0, ctx->stream>>>(
-                (const uint64_t*)keys, idx, n, verts, t->parent, t->left, t->right, t->data,
-                t->bbmin, t->bbmax, t->area, t->span, slots);
+            k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, verts, t->parent, t->left,
+                                                   t->right, t->data, t->bbmin, t->bbmax, t->area,
+                                                   t->span, slots);
         });
     }
+    SingleKeys<uint32_t> kv{(const uint32_t*)keys, (int)n - 1};
     return launch(ctx, "build_refit30", bytes, [&] {
-        k_build<uint32_t><<<grid, 256, 0, ctx->stream>>>((const uint32_t*)keys, idx, n, verts,
-                                                         t->parent, t->left, t->right, t->data,
-                                                         t->bbmin, t->bbmax, t->area, t->span,
-                                                         slots);
+        k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, verts, t->parent, t->left,
+                                               t->right, t->data, t->bbmin, t->bbmax, t->area,
+                                               t->span, slots);
+    });
+}
+
+atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32_t* idx,
+                            const float4* verts, atr_tree* t, const uint32_t* d_seg,
+                            uint64_t* slots)
+{
+    const uint32_t n = t->n;
+    ForestKeys kv{keys, (int)n - 1, d_seg};
+    return launch(ctx, "build_refit_forest", (double)n * 176.0, [&] {
+        k_build<<<div_up(n, 256), 256, 0, ctx->stream>>>(
+            kv, idx, n, (int)t->first_leaf, verts, t->parent, t->left, t->right, t->data,
+            t->bbmin, t->bbmax, t->area, t->span, slots);
     });
 }
 

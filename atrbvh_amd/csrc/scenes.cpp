@@ -1,6 +1,9 @@
 // Seeded synthetic scenes (see include/atrbvh_scenes.h).
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "../../include/atrbvh_scenes.h"
@@ -104,6 +107,35 @@ extern "C" void atr_gen_gmm(uint64_t seed, uint32_t n, uint32_t clusters, float 
             p[3] = 0.0f;
         }
     }
+}
+
+extern "C" void atr_gen_batch_sizes(uint64_t seed, uint32_t meshes, uint32_t lo, uint32_t hi,
+                                    uint32_t* offsets)
+{
+    Pcg32 g(seed);
+    const double llo = std::log((double)lo), lhi = std::log((double)hi);
+    offsets[0] = 0;
+    for (uint32_t m = 0; m < meshes; ++m) {
+        double k = std::floor(std::exp(llo + (lhi - llo) * g.uniform()) + 0.5);
+        k = std::min(std::max(k, (double)lo), (double)hi);
+        offsets[m + 1] = offsets[m] + (uint32_t)k;
+    }
+}
+
+extern "C" void atr_gen_soup_batch(uint64_t seed_base, uint32_t meshes, const uint32_t* offsets,
+                                   float sigma, float* v)
+{
+    std::atomic<uint32_t> next{0};
+    auto work = [&] {
+        for (uint32_t m = next++; m < meshes; m = next++)
+            atr_gen_soup(seed_base + m, offsets[m + 1] - offsets[m], sigma,
+                         v + (size_t)offsets[m] * 12);
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < hw; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
 }
 
 extern "C" int atr_trbvh_schedule(int treelet, int warp, int32_t* out, int cap)

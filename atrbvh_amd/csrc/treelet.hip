@@ -49,7 +49,7 @@ struct HalfTreelet {
 // non-power-of-two halving tree.
 template <bool COH>
 __device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int root, int n,
-                                             uint32_t N, const int32_t* left,
+                                             int first_leaf, const int32_t* left,
                                              const int32_t* right, const float* area)
 {
     if (hl == 0) {
@@ -67,7 +67,7 @@ __device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int r
         float a = -INFINITY;
         if (hl < m) {
             a = h.area[hl];
-            if (h.leaf[hl] >= (int)N - 1) a = -FLT_MAX;
+            if (h.leaf[hl] >= first_leaf) a = -FLT_MAX;
         }
         int cnt = m;
         int sh = m >> 1;
@@ -101,7 +101,7 @@ constexpr int kAggFixedWords = 32 + 32 + 32 + 192 + 64;
 
 template <bool COH>
 __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, float root_sah, int n,
-                                               uint32_t N, float ci, int32_t* parent,
+                                               int first_leaf, float ci, int32_t* parent,
                                                int32_t* left, int32_t* right, float4* bbmin,
                                                float4* bbmax, float* area, float* nsah,
                                                int2* span)
@@ -110,7 +110,7 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
     float* box = lds + 96;
     int2* sp = (int2*)(lds + 288);
     float* D = lds + kAggFixedWords;
-    form_treelet<COH>(h, hl, root, n, N, left, right, area);
+    form_treelet<COH>(h, hl, root, n, first_leaf, left, right, area);
 
     int node = -1;
     float sah = 0.0f;
@@ -262,7 +262,7 @@ constexpr int kTrWords = 8 + 8 + 8 + 48 + 16 + kTrSub * 6 + kTrSub * 2 + kTrSub 
 
 template <bool COH>
 __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, float root_sah,
-                                              int root_tris, uint32_t N, float ci, float ct,
+                                              int root_tris, int first_leaf, float ci, float ct,
                                               const int32_t* sched, int rounds, int32_t* parent,
                                               int32_t* left, int32_t* right, float4* bbmin,
                                               float4* bbmax, float* area, float* nsah,
@@ -277,7 +277,7 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
     int32_t* mask = (int32_t*)(cost + kTrSub);
     int32_t* stk = mask + kTrSub;
     // formTreelet writes at most index n-1 = 6 into the 8-entry arrays
-    form_treelet<COH>(h, hl, root, kTrN, N, left, right, area);
+    form_treelet<COH>(h, hl, root, kTrN, first_leaf, left, right, area);
 
     if (hl < kTrN) {
         const int node = h.leaf[hl];
@@ -487,7 +487,8 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
 
 // ------------------------------------------------------------------ driver
 template <bool AGG>
-__global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gamma, float ci,
+__global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int first_leaf, int n,
+                                                      int gamma, float ci,
                                                       float ct, int32_t* parent, int32_t* left,
                                                       int32_t* right, float4* bbmin,
                                                       float4* bbmax, float* area, int32_t* tris,
@@ -508,8 +509,8 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gam
     s_cnt[tid] = 0u;
     __syncthreads();
     int cur = -1;
-    if (gid < N) {
-        const int leaf = (int)(N - 1 + gid);
+    if (gid < n_leaves) {
+        const int leaf = first_leaf + (int)gid;
         st_coh(&tris[leaf], 1);
         st_coh(&nsah[leaf], ct * area[leaf]);
         cur = parent[leaf];
@@ -569,18 +570,18 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t N, int n, int gam
             if (my_r >= 0) {
                 if (AGG) {
                     if (root_local)
-                        atrbvh_treelet<false>(lds, hl, root, root_sah, n, N, ci, parent, left,
+                        atrbvh_treelet<false>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
                                               right, bbmin, bbmax, area, nsah, span);
                     else
-                        atrbvh_treelet<true>(lds, hl, root, root_sah, n, N, ci, parent, left,
+                        atrbvh_treelet<true>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
                                              right, bbmin, bbmax, area, nsah, span);
                 } else {
                     if (root_local)
-                        trbvh_treelet<false>(lds, hl, root, root_sah, root_tris, N, ci, ct, sched,
+                        trbvh_treelet<false>(lds, hl, root, root_sah, root_tris, first_leaf, ci, ct, sched,
                                              rounds, parent, left, right, bbmin, bbmax, area,
                                              nsah, span);
                     else
-                        trbvh_treelet<true>(lds, hl, root, root_sah, root_tris, N, ci, ct, sched,
+                        trbvh_treelet<true>(lds, hl, root, root_sah, root_tris, first_leaf, ci, ct, sched,
                                             rounds, parent, left, right, bbmin, bbmax, area,
                                             nsah, span);
                 }
@@ -612,13 +613,14 @@ atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma
                            float ct, int32_t* tris, float* nsah, uint32_t* counters)
 {
     const uint32_t N = t->n;
-    ATR_TRY(zero_counters(ctx, counters, N - 1));
+    const int first_leaf = (int)t->first_leaf;
+    ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
     const int half_words = kAggFixedWords + treelet * (treelet - 1) / 2;
     const size_t lds = (size_t)half_words * 8 * sizeof(float);
     return launch(ctx, "atrbvh_pass", 0.0, [&] {
         k_treelet_pass<true><<<div_up(N, 256), 256, lds, ctx->stream>>>(
-            N, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax, t->area,
-            tris, nsah, counters, t->span, nullptr, 0, half_words);
+            N, first_leaf, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin,
+            t->bbmax, t->area, tris, nsah, counters, t->span, nullptr, 0, half_words);
     });
 }
 
@@ -627,12 +629,13 @@ atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, fl
                           const int32_t* d_schedule, int rounds)
 {
     const uint32_t N = t->n;
-    ATR_TRY(zero_counters(ctx, counters, N - 1));
+    const int first_leaf = (int)t->first_leaf;
+    ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
     const size_t lds = (size_t)kTrWords * 8 * sizeof(float);
     return launch(ctx, "trbvh_pass", 0.0, [&] {
         k_treelet_pass<false><<<div_up(N, 256), 256, lds, ctx->stream>>>(
-            N, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax, t->area,
-            tris, nsah, counters, t->span, d_schedule, rounds, kTrWords);
+            N, first_leaf, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax,
+            t->area, tris, nsah, counters, t->span, d_schedule, rounds, kTrWords);
     });
 }
 

@@ -55,35 +55,100 @@ def build_step(scene, treelet="atrbvh", want_sah=False):
     return tree, collapsed, sah
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the roofline kernel measured with rocprofv3 PMC
-    counters (committed summary), or None when no such summary exists."""
-    path = os.path.join(ROOT, "profiles", "onesweep64_c2_pmc.json")
-    if not os.path.exists(path):
-        return None
+PMC_NAMES = {"build_refit63": "k_build<SingleKeys<unsigned long> >",
+             "onesweep64": "k_onesweep<unsigned long>", "encode63": "k_encode<unsigned long>"}
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_build_pmc_traffic.json")
     try:
         with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
+            k = json.load(f)["kernels"][PMC_NAMES[kernel]]
+        return round(float(k["hbm_bytes_per_launch"]))
     except Exception:
         return None
 
 
-def cpu_baseline(sample_tris: int):
-    """The oracle (serial C restatement of the reference build path) on a
-    bounded sample of the same workload, on one host core."""
+def cpu_baseline(sample_tris: int, min_seconds: float = 10.0):
+    """The oracle (serial C restatement of the reference build path) on the same
+    workload, on one host core, repeated until at least min_seconds of work."""
     import oracle
 
     v = ab.gen_gmm(SEED, sample_tris)
     mn, mx = oracle.scene_bbox(v)
-    t0 = time.perf_counter()
-    t = oracle.lbvh(v, mn, mx, 63)
-    oracle.atrbvh(t, 9, 2)
-    oracle.collapse(t)
-    dt = time.perf_counter() - t0
+    runs = []
+    while sum(runs) < min_seconds:
+        t0 = time.perf_counter()
+        t = oracle.lbvh(v, mn, mx, 63)
+        oracle.atrbvh(t, 9, 2)
+        oracle.collapse(t)
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
     return {"value": round(sample_tris / dt / 1e6, 4), "unit": "Mtris/s", "cores": 1,
             "kind": "port",
-            "sample": f"oracle LBVH63 + ATRBVH(9,2) + collapse of a {sample_tris:,}-triangle "
-                      f"GMM scene from the same generator, single thread, {dt:.1f} s"}
+            "sample": f"oracle LBVH63 + ATRBVH(9,2) + collapse of the {sample_tris:,}-triangle "
+                      f"GMM scene, single thread, median of {len(runs)} runs "
+                      f"({sum(runs):.1f} s total)"}
+
+
+def batched_leg(ctx, meshes: int, reps: int):
+    """Batched builds (4,096 soups of 64..65,536 triangles, ~39M triangles,
+    per-mesh seeds 3000 + m, ATRBVH 9/2, 30-bit keys) as one captured launch
+    sequence, against the plain per-mesh loop of the same library calls."""
+    off = ab.gen_batch_sizes(3, meshes, 64, 65536)
+    v = ab.gen_soup_batch(3000, off)
+    total = int(off[-1])
+    b = ab.Batch(ctx, v, off)
+    del v
+    b.build(9, 2, graph=True)  # capture + first replay
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.build(9, 2, graph=True)
+    ctx.sync()
+    graph_ms = (time.perf_counter() - t0) * 1000.0 / reps
+    t0 = time.perf_counter()
+    b.build(9, 2, graph=False)
+    ctx.sync()
+    eager_ms = (time.perf_counter() - t0) * 1000.0
+    sah_batch = b.sah()
+    base = b.device_vertices()
+    scenes = [ab.Scene.from_device(ctx, base + 48 * int(off[m]), int(off[m + 1] - off[m]))
+              for m in range(meshes)]
+    ctx.sync()
+    # plain loop: one untimed pass fills the allocator cache with every mesh
+    # size, so the timed pass allocates nothing either
+    for s in scenes:
+        t = ab.LBVHBuilder(False).BuildTree(s)
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+        t.close()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s in scenes:
+        t = ab.LBVHBuilder(False).BuildTree(s)
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+        t.close()
+    ctx.sync()
+    loop_ms = (time.perf_counter() - t0) * 1000.0
+    same = True
+    for m, s in enumerate(scenes):
+        t = ab.LBVHBuilder(False).BuildTree(s)
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+        same &= np.float32(t.SAH()).tobytes() == sah_batch[m].tobytes()
+        t.close()
+        s.close()
+    b.close()
+    return {"meshes": meshes, "triangles": total,
+            "graph_ms": round(graph_ms, 3), "eager_ms": round(eager_ms, 3),
+            "builds_per_s": round(meshes / (graph_ms * 1e-3), 1),
+            "Mtris_s": round(total / (graph_ms * 1e-3) / 1e6, 1),
+            "plain_loop_ms": round(loop_ms, 3),
+            "plain_builds_per_s": round(meshes / (loop_ms * 1e-3), 1),
+            "speedup_vs_plain": round(loop_ms / graph_ms, 2),
+            "sah_identical": bool(same),
+            "mean_sah": round(float(np.mean(sah_batch)), 4)}
 
 
 def copy_calibration(nbytes: int = 256 << 20):
@@ -105,6 +170,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-tris", type=int, default=N_TRIS)
     ap.add_argument("--no-trace", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-build leg")
+    ap.add_argument("--batch-meshes", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -170,7 +237,7 @@ def main():
     achieved = per_launch_bytes / per_launch_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(),
+                "traffic": traffic_from_profiles(dom),
                 "algorithmic_bytes_per_launch": per_launch_bytes,
                 "avg_launch_us": round(per_launch_s * 1e6, 3),
                 "streaming_group": {"kernels": list(streaming), "bytes_per_step": s_bytes,
@@ -245,6 +312,14 @@ def main():
     collapsed.close()
     tree.close()
 
+    batched = None
+    if not args.no_batch:
+        log(f"[bench] batched leg: {args.batch_meshes} meshes")
+        batched = batched_leg(ctx, args.batch_meshes, reps=3)
+        log(f"[bench] batched {batched['graph_ms']} ms ({batched['builds_per_s']} builds/s), "
+            f"plain loop {batched['plain_loop_ms']} ms; x{batched['speedup_vs_plain']}, "
+            f"SAH identical: {batched['sah_identical']}")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline on {args.cpu_tris:,} triangles")
@@ -274,6 +349,7 @@ def main():
         "trbvh_build": {"ms": round(trbvh_ms, 3),
                         "Mtris_s": round(N_TRIS / (trbvh_ms * 1e-3) / 1e6, 2)},
         "trace": trace,
+        "batched": batched,
         "gpu_ms_per_step": round(gpu_ms, 4),
         "kernels": kernels,
         "roofline": roofline,

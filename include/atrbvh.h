@@ -215,6 +215,39 @@ atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t
  * with the topology left unchanged. */
 atr_status atr_refit(atr_tree* tree, atr_scene* scene);
 
+/* ---------------------------------------------------------- batched builds */
+/* Many independent meshes built together.  No reference counterpart: it
+ * replaces a host loop of LBVHBuilder::BuildTree (30-bit keys) +
+ * AgglomerativeTreeletOptimizer::Optimize over meshes
+ * (BVHRT-Core/src/LBVHBuilder.cpp:21-130,
+ *  AgglomerativeTreeletOptimizer.cpp:24-110) with one launch sequence over
+ * all meshes; every mesh's tree is bit-identical to the loop's.
+ * vertices: float4[3T] mesh-major (host, or device for _create_device);
+ * tri_offsets: host uint32[num_meshes + 1] prefix offsets, tri_offsets[0] = 0,
+ * every mesh >= 2 triangles, T < 2^30; bboxes: host float[6 * num_meshes]
+ * (min xyz, max xyz per mesh) or NULL => every build computes each mesh's box
+ * on the device with SceneConverter's rule. */
+typedef struct atr_batch atr_batch;
+atr_status atr_batch_create(atr_context* ctx, const float* vertices, const uint32_t* tri_offsets,
+                            uint32_t num_meshes, const float* bboxes, atr_batch** out);
+atr_status atr_batch_create_device(atr_context* ctx, const float* d_vertices,
+                                   const uint32_t* tri_offsets, uint32_t num_meshes,
+                                   const float* bboxes, atr_batch** out);
+/* Device pointer of the batch's float4 vertex array (mesh-major); callers may
+ * move vertices in place between builds. */
+atr_status atr_batch_vertices(atr_batch* b, float** d_vertices);
+/* LBVH (30-bit) + ATRBVH(treelet, iterations) of every mesh, asynchronous.
+ * use_graph != 0: the launch sequence is captured once into a HIP graph (per
+ * parameter set) and replayed; it allocates nothing after the first call. */
+atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float ci, float ct,
+                           int use_graph);
+/* SoABVHTree::SAH of every mesh (same summation order), into host out[num_meshes].
+ * Blocking. */
+atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out);
+/* Mesh `mesh` as a standalone tree in the BVHRT layout (caller destroys it). */
+atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out);
+atr_status atr_batch_destroy(atr_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

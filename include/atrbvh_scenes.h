@@ -29,6 +29,16 @@ void atr_gen_soup(uint64_t seed, uint32_t n, float sigma, float* vertices);
 void atr_gen_gmm(uint64_t seed, uint32_t n, uint32_t clusters, float sigma_lo, float sigma_hi,
                  float edge_median, float edge_sigma, float* vertices);
 
+/* Mesh sizes of a batch: N_m = round(exp(U[ln lo, ln hi])) clamped to
+ * [lo, hi]; writes offsets[0..meshes] (prefix sums, offsets[0] = 0). */
+void atr_gen_batch_sizes(uint64_t seed, uint32_t meshes, uint32_t lo, uint32_t hi,
+                         uint32_t* offsets);
+
+/* Batch of soups: mesh m is atr_gen_soup(seed_base + m, N_m, sigma) written at
+ * triangle offsets[m] (host threads, deterministic). */
+void atr_gen_soup_batch(uint64_t seed_base, uint32_t meshes, const uint32_t* offsets, float sigma,
+                        float* vertices);
+
 /* TRBVH lane schedule (the library's own generator): writes rounds*warp ints
  * into out when cap allows and returns the number of rounds, or -1. */
 int atr_trbvh_schedule(int treelet, int warp, int32_t* out, int cap);

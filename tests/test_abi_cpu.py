@@ -111,3 +111,28 @@ def test_cpp_facade_compiles_and_reports_errors(tmp_path):
     rc, out, res = run_compat_driver(root, ab.gen_soup(1, 64), False, "atrbvh", str(tmp_path))
     assert rc == 3 and res is None
     assert out.startswith("status 3:")
+
+
+def test_batch_argument_validation_without_device():
+    L = ab.lib()
+    out = ctypes.c_void_p()
+    off = np.array([0, 5, 9], np.uint32)
+    v = np.zeros((27, 4), np.float32)
+    p32 = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    assert L.atr_batch_create(None, v.ctypes.data, p32, 2, None, ctypes.byref(out)) == 1
+    assert L.atr_batch_build(None, 9, 2, 1.2, 1.0, 1) == 1
+    assert L.atr_batch_destroy(None) == 0
+
+
+def test_batch_generators():
+    off = ab.gen_batch_sizes(3, 4096, 64, 65536)
+    sizes = np.diff(off.astype(np.int64))
+    assert sizes.min() >= 64 and sizes.max() <= 65536
+    assert 30e6 < off[-1] < 50e6  # about 40M triangles
+    # log-uniform: the median size is near the geometric mean of the range
+    assert 1500 < np.median(sizes) < 3000
+    small = off[:9]
+    v = ab.gen_soup_batch(3000, small)
+    for m in range(8):
+        vm = v[3 * int(small[m]): 3 * int(small[m + 1])]
+        assert np.array_equal(vm, ab.gen_soup(3000 + m, int(small[m + 1] - small[m])))
