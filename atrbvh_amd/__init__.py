@@ -138,6 +138,12 @@ def lib():
         L.atr_batch_sah.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _f32p]
         L.atr_batch_extract.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]
         L.atr_batch_destroy.argtypes = [_vp]
+        L.atr_batch_sah_device.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _vp]
+        L.atr_tree_sah_device.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _vp]
+        L.atr_hit_adjoint.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _vp,
+                                      ctypes.c_uint32, _vp]
+        L.atr_depth_loss.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]
+        L.atr_mesh_step.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_float, _vp, _vp]
         _lib = L
     return _lib
 
@@ -285,6 +291,25 @@ class Context:
         check(lib().atr_sort_pairs(self.h, key_bits, d_keys.ptr,
                                    d_vals.ptr if d_vals is not None else None, n))
 
+    # differentiable hits
+    def hit_adjoint(self, scene: "Scene", d_rays: DeviceBuffer, d_hits: DeviceBuffer, n: int,
+                    d_grad_tuv: DeviceBuffer, d_index: DeviceBuffer | None, n_grad: int,
+                    d_grad: DeviceBuffer):
+        check(lib().atr_hit_adjoint(self.h, scene.h, d_rays.ptr, d_hits.ptr, n, d_grad_tuv.ptr,
+                                    d_index.ptr if d_index is not None else None, n_grad,
+                                    d_grad.ptr))
+
+    def depth_loss(self, d_hits: DeviceBuffer, d_target: DeviceBuffer, n: int,
+                   d_grad_tuv: DeviceBuffer, d_loss: DeviceBuffer, offset: int = 0):
+        check(lib().atr_depth_loss(self.h, d_hits.ptr, d_target.ptr, n, d_grad_tuv.ptr,
+                                   d_loss.ptr + 8 * offset))
+
+    def mesh_step(self, d_mesh: DeviceBuffer, n_vertices: int, d_grad: DeviceBuffer | None,
+                  lr: float, d_index: DeviceBuffer, scene: "Scene"):
+        check(lib().atr_mesh_step(self.h, d_mesh.ptr, n_vertices,
+                                  d_grad.ptr if d_grad is not None else None, lr, d_index.ptr,
+                                  scene.h))
+
     def rays_pinhole(self, eye, basis, tan_half: float, w: int, h: int, tmax: float,
                      d_rays: DeviceBuffer):
         e = np.asarray(eye, np.float32)
@@ -387,6 +412,10 @@ class Tree:
         out = ctypes.c_float()
         check(lib().atr_tree_sah(self.h, ci, ct, ctypes.byref(out)))
         return out.value
+
+    def sah_device(self, d_out: DeviceBuffer, offset: int = 0, ci: float = 1.2,
+                   ct: float = 1.0):
+        check(lib().atr_tree_sah_device(self.h, ci, ct, d_out.ptr + 4 * offset))
 
     def refit(self, scene: Scene | None = None):
         check(lib().atr_refit(self.h, scene.h if scene is not None else None))

@@ -39,7 +39,8 @@ DEVICE = [
     "-packed-fp32-ops",
 ]
 
-SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "batch.hip", "api.cpp",
+SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "batch.hip", "adjoint.hip",
+           "api.cpp",
            "scenes.cpp", "schedule.cpp"]
 
 

@@ -883,6 +883,50 @@ atr_status atr_refit(atr_tree* t, atr_scene* scene)
     return s;
 }
 
+// ------------------------------------------------------- differentiation
+atr_status atr_hit_adjoint(atr_context* ctx, atr_scene* scene, const float* d_rays,
+                           const int32_t* d_hits, uint32_t num_rays, const float* d_grad_tuv,
+                           const uint32_t* d_index, uint32_t num_grad_vertices, float* d_grad)
+{
+    if (!ctx || !scene || !d_rays || !d_hits || !d_grad_tuv || !d_grad)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (scene->ctx != ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "scene of another context");
+    if (!d_index && (uint64_t)num_grad_vertices < 3ull * scene->n)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "without an index the gradient needs 3 slots per triangle");
+    if (num_rays == 0) return ATR_OK;
+    return run_hit_adjoint(ctx, scene->verts, scene->n, (const float4*)d_rays,
+                           (const int4*)d_hits, num_rays, d_grad_tuv, d_index, num_grad_vertices,
+                           d_grad);
+}
+
+atr_status atr_depth_loss(atr_context* ctx, const int32_t* d_hits, const int32_t* d_target_hits,
+                          uint32_t num_rays, float* d_grad_tuv, double* d_loss)
+{
+    if (!ctx || !d_hits || !d_target_hits || !d_grad_tuv || !d_loss)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (num_rays == 0) return ATR_OK;
+    return run_depth_loss(ctx, (const int4*)d_hits, (const int4*)d_target_hits, num_rays,
+                          d_grad_tuv, d_loss);
+}
+
+atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_vertices,
+                         float* d_grad, float lr, const uint32_t* d_index, atr_scene* scene)
+{
+    if (!ctx || !d_mesh_vertices || !d_index || !scene)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (scene->ctx != ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "scene of another context");
+    if (num_vertices == 0) return fail(ATR_ERR_INVALID_ARGUMENT, "empty mesh");
+    return run_mesh_step(ctx, (float4*)d_mesh_vertices, num_vertices, d_grad, lr, d_index,
+                         scene->n, scene->verts);
+}
+
+atr_status atr_tree_sah_device(atr_tree* t, float ci, float ct, float* d_out)
+{
+    if (!t || !d_out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    return run_sah_parallel(t->ctx, nullptr, 1, t->n, (int)t->first_leaf, t->area, ci, ct, d_out);
+}
+
 // --------------------------------------------------------- batched builds
 static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_device,
                                const uint32_t* tri_offsets, uint32_t meshes,
@@ -1080,6 +1124,14 @@ atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out)
                            ctx->stream));
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     return ATR_OK;
+}
+
+atr_status atr_batch_sah_device(atr_batch* b, float ci, float ct, float* d_out)
+{
+    if (!b || !d_out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
+    return run_sah_parallel(b->ctx, b->d_seg, b->meshes, 0, (int)b->forest->first_leaf,
+                            b->forest->area, ci, ct, d_out);
 }
 
 atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out)

@@ -62,7 +62,51 @@ __global__ __launch_bounds__(64) void k_forest_sah(const uint32_t* __restrict__ 
     out[m] = (ci * si + ct * sl) / total;
 }
 
+// SAH of every mesh with double-precision accumulation, one block per mesh
+// (for monitoring, e.g. a rebuild trigger; the summation order differs from
+// SoABVHTree::SAH, so the last bits may too).  Mesh m: internal nodes
+// [seg[m] + 1, seg[m] + Nm - 2] (root seg[m] excluded), leaves
+// first_leaf + [seg[m], seg[m+1]).  A single tree of n leaves passes
+// seg = NULL (one mesh [0, n)) and first_leaf = n - 1.
+__global__ __launch_bounds__(256) void k_sah_parallel(const uint32_t* __restrict__ seg,
+                                                      uint32_t n_single, int first_leaf,
+                                                      const float* __restrict__ area, float ci,
+                                                      float ct, float* out)
+{
+    __shared__ double s_i[4], s_l[4];
+    const uint32_t m = blockIdx.x;
+    const uint32_t s = seg ? seg[m] : 0u, e = seg ? seg[m + 1] : n_single;
+    double si = 0.0, sl = 0.0;
+    for (uint32_t g = s + 1 + threadIdx.x; g + 1 < e; g += 256u) si += area[g];
+    for (uint32_t p = s + threadIdx.x; p < e; p += 256u) sl += area[first_leaf + p];
+    for (int o = 32; o > 0; o >>= 1) {
+        si += __shfl_xor(si, o, 64);
+        sl += __shfl_xor(sl, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_i[threadIdx.x >> 6] = si;
+        s_l[threadIdx.x >> 6] = sl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double root = area[s];
+        const double ti = root + ((s_i[0] + s_i[1]) + (s_i[2] + s_i[3]));
+        const double tl = (s_l[0] + s_l[1]) + (s_l[2] + s_l[3]);
+        out[m] = (float)(((double)ci * ti + (double)ct * tl) / root);
+    }
+}
+
 }  // namespace
+
+atr_status run_sah_parallel(atr_context* ctx, const uint32_t* d_seg, uint32_t meshes,
+                            uint32_t n_single, int first_leaf, const float* area, float ci,
+                            float ct, float* d_out)
+{
+    return launch(ctx, "sah_parallel", 0.0, [&] {
+        k_sah_parallel<<<meshes, 256, 0, ctx->stream>>>(d_seg, n_single, first_leaf, area, ci, ct,
+                                                         d_out);
+    });
+}
 
 atr_status run_forest_extract(atr_context* ctx, const atr_tree* f, uint32_t s, uint32_t nm,
                               atr_tree* t)

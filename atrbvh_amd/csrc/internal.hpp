@@ -226,6 +226,18 @@ atr_status run_forest_extract(atr_context* ctx, const atr_tree* f, uint32_t s, u
                               atr_tree* t);
 atr_status run_forest_sah(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,
                           uint32_t meshes, float ci, float ct, float* d_out);
+atr_status run_sah_parallel(atr_context* ctx, const uint32_t* d_seg, uint32_t meshes,
+                            uint32_t n_single, int first_leaf, const float* area, float ci,
+                            float ct, float* d_out);
+
+// Differentiable hits (adjoint.hip)
+atr_status run_hit_adjoint(atr_context* ctx, const float4* v, uint32_t n_tris,
+                           const float4* rays, const int4* hits, uint32_t R, const float* g_tuv,
+                           const uint32_t* index, uint32_t n_grad, float* grad);
+atr_status run_depth_loss(atr_context* ctx, const int4* hits, const int4* target, uint32_t R,
+                          float* g_tuv, double* loss);
+atr_status run_mesh_step(atr_context* ctx, float4* mesh, uint32_t V, float* grad, float lr,
+                         const uint32_t* index, uint32_t n_tris, float4* soup);
 
 // Treelets (treelet.hip)
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,

@@ -151,6 +151,79 @@ def batched_leg(ctx, meshes: int, reps: int):
             "mean_sah": round(float(np.mean(sah_batch)), 4)}
 
 
+def differentiable_leg(ctx, steps: int = 50, lr: float = 0.02):
+    """Refit-driven optimisation of the 262,144-triangle perturbed sphere
+    toward the unit sphere's depths under 1024 x 1024 orthographic rays: per
+    step trace -> depth loss -> hit adjoint (atomic scatter-add over the fixed
+    topology) -> vertex step + soup gather -> refit -> device SAH check
+    (ATRBVH rebuild when it exceeds 1.10x the post-build SAH) -> collapse."""
+    import atrbvh_amd.workloads as wl
+
+    m = wl.sphere_mesh(seed=4)
+    rays = wl.ortho_rays(1024)
+    R = rays.shape[0]
+    V = m["vertices"].shape[0]
+    s = ctx.scene(wl.soup(m["vertices"], m["index"]))
+    st = ctx.scene(wl.soup(m["unit"], m["index"]))
+    d_rays = ctx.to_device(rays)
+    d_hits = ctx.buffer(R * 16)
+    d_target = ctx.buffer(R * 16)
+    d_g = ctx.buffer(R * 12)
+    d_index = ctx.to_device(m["index"].astype(np.uint32).reshape(-1))
+    d_mesh = ctx.to_device(wl.mesh_float4(m["vertices"]))
+    d_grad = ctx.to_device(np.zeros((V, 3), np.float32))
+    d_loss = ctx.to_device(np.zeros(steps, np.float64))
+    d_sah = ctx.buffer(4 * steps)
+
+    def build(scene):
+        t = ab.LBVHBuilder(False).BuildTree(scene)
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+        return t
+
+    tt = build(st)
+    ct, _ = ab.BVHTreeCollapser().Collapse(tt)
+    ct.trace(d_rays, R, d_target)
+    ct.close()
+    tt.close()
+    t = build(s)
+    sah_built = t.SAH()
+    c, _ = ab.BVHTreeCollapser().Collapse(t)
+    rebuilds = 0
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        c.trace(d_rays, R, d_hits)
+        ctx.depth_loss(d_hits, d_target, R, d_g, d_loss, offset=k)
+        ctx.hit_adjoint(s, d_rays, d_hits, R, d_g, d_index, V, d_grad)
+        ctx.mesh_step(d_mesh, V, d_grad, lr, d_index, s)
+        t.refit(s)
+        t.sah_device(d_sah, offset=k)
+        if float(d_sah.download(np.float32, steps)[k]) > 1.10 * sah_built:
+            t.close()
+            t = build(s)
+            sah_built = t.SAH()
+            rebuilds += 1
+        c.close()
+        c, _ = ab.BVHTreeCollapser().Collapse(t)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    losses = d_loss.download(np.float64, steps)
+    sah_refit = t.SAH()
+    fresh = build(s)
+    sah_fresh = fresh.SAH()
+    fresh.close()
+    c.close()
+    t.close()
+    s.close()
+    st.close()
+    return {"triangles": int(m["index"].shape[0]), "rays": R, "steps": steps,
+            "ms_per_step": round(dt * 1000.0 / steps, 3), "steps_per_s": round(steps / dt, 2),
+            "loss_first": round(float(losses[0]), 4), "loss_last": round(float(losses[-1]), 4),
+            "rebuilds": rebuilds, "sah_refit_final": round(sah_refit, 4),
+            "sah_rebuilt_final": round(sah_fresh, 4),
+            "sah_drift": round(sah_refit / sah_fresh - 1.0, 5)}
+
+
 def copy_calibration(nbytes: int = 256 << 20):
     import torch
 
@@ -172,6 +245,7 @@ def main():
     ap.add_argument("--no-trace", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-build leg")
     ap.add_argument("--batch-meshes", type=int, default=4096)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -320,6 +394,13 @@ def main():
             f"plain loop {batched['plain_loop_ms']} ms; x{batched['speedup_vs_plain']}, "
             f"SAH identical: {batched['sah_identical']}")
 
+    adjoint = None
+    if not args.no_adjoint:
+        log("[bench] differentiable refit leg: 50 steps")
+        adjoint = differentiable_leg(ctx)
+        log(f"[bench] adjoint loop {adjoint['ms_per_step']} ms/step, loss "
+            f"{adjoint['loss_first']} -> {adjoint['loss_last']}, rebuilds {adjoint['rebuilds']}")
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline on {args.cpu_tris:,} triangles")
@@ -350,6 +431,7 @@ def main():
                         "Mtris_s": round(N_TRIS / (trbvh_ms * 1e-3) / 1e6, 2)},
         "trace": trace,
         "batched": batched,
+        "differentiable": adjoint,
         "gpu_ms_per_step": round(gpu_ms, 4),
         "kernels": kernels,
         "roofline": roofline,

@@ -215,6 +215,37 @@ atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t
  * with the topology left unchanged. */
 atr_status atr_refit(atr_tree* tree, atr_scene* scene);
 
+/* ------------------------------------------------------- differentiation */
+/* Adjoint of closest hits (no reference counterpart; the reference's tracer
+ * is not differentiable).  For every ray r whose hit d_hits[4r] (as written
+ * by atr_trace) is a triangle of `scene`, with upstream gradient
+ * d_grad_tuv[3r..3r+2] = dL/d(t, u, v), adds dL/dv_i = -b_i M^{-T} g to the
+ * gradient of the triangle's three vertices (b = (1-u-v, u, v),
+ * M = [-d | v1-v0 | v2-v0]) with atomic adds into d_grad (float[3 x
+ * num_grad_vertices], accumulated, not cleared).  d_index (uint32[3n],
+ * nullable) maps triangle corners to mesh vertices (fixed topology); NULL =>
+ * slot 3*triangle + corner.  Corners mapped at or beyond num_grad_vertices
+ * are skipped. */
+atr_status atr_hit_adjoint(atr_context* ctx, atr_scene* scene, const float* d_rays,
+                           const int32_t* d_hits, uint32_t num_rays, const float* d_grad_tuv,
+                           const uint32_t* d_index, uint32_t num_grad_vertices, float* d_grad);
+/* Depth loss of the differentiable-refit workload: adds sum (t - t_target)^2
+ * over rays hit in both d_hits and d_target_hits to *d_loss (device double)
+ * and writes d_grad_tuv = (2 (t - t_target), 0, 0) per ray (zero elsewhere). */
+atr_status atr_depth_loss(atr_context* ctx, const int32_t* d_hits, const int32_t* d_target_hits,
+                          uint32_t num_rays, float* d_grad_tuv, double* d_loss);
+/* Vertex step on an indexed mesh: d_mesh_vertices (float4[num_vertices])
+ * -= lr * d_grad (float[3 num_vertices], then cleared; d_grad NULL => no
+ * step), then the scene's triangle soup is gathered from the mesh through
+ * d_index (uint32[3n]).  Follow with atr_refit to update the tree. */
+atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_vertices,
+                         float* d_grad, float lr, const uint32_t* d_index, atr_scene* scene);
+
+/* SAH of a tree into device memory, asynchronously, accumulated in double
+ * precision by a parallel reduction (for monitoring such as a rebuild
+ * trigger; atr_tree_sah gives the reference's exact fp32 summation order). */
+atr_status atr_tree_sah_device(atr_tree* tree, float ci, float ct, float* d_out);
+
 /* ---------------------------------------------------------- batched builds */
 /* Many independent meshes built together.  No reference counterpart: it
  * replaces a host loop of LBVHBuilder::BuildTree (30-bit keys) +
@@ -244,6 +275,9 @@ atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float
 /* SoABVHTree::SAH of every mesh (same summation order), into host out[num_meshes].
  * Blocking. */
 atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out);
+/* Per-mesh SAH into device memory d_out[num_meshes], asynchronously, with
+ * the double-precision parallel reduction of atr_tree_sah_device. */
+atr_status atr_batch_sah_device(atr_batch* b, float ci, float ct, float* d_out);
 /* Mesh `mesh` as a standalone tree in the BVHRT layout (caller destroys it). */
 atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out);
 atr_status atr_batch_destroy(atr_batch* b);

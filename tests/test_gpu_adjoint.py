@@ -1,0 +1,167 @@
+"""GPU checks of the hit adjoint on the differentiable workload (262,144-triangle
+perturbed sphere, 1024 x 1024 orthographic rays): fp32 scatter-added vertex
+gradients against the FP64 host gradient of the same hits, against FP64
+central differences (h = 1e-4) on 1,024 random vertices, and the generic
+(t, u, v) adjoint against its FP64 counterpart.  Bar: 1e-4 relative (norm over
+the checked vertices)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import atrbvh_amd as ab
+import atrbvh_amd.workloads as wl
+from adjoint_ref import (fd_fixed, fd_retrace, loss_grad_fp64, rel_err, target_depth,
+                         tuv_adjoint_fp64, well_conditioned_vertices)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def sphere(ctx):
+    m = wl.sphere_mesh(seed=4)
+    rays = wl.ortho_rays(1024)
+    v32 = m["vertices"].astype(np.float32).astype(np.float64)
+    u32 = m["unit"].astype(np.float32).astype(np.float64)
+    s = ctx.scene(wl.soup(m["vertices"], m["index"]))
+    st = ctx.scene(wl.soup(m["unit"], m["index"]))
+    d_rays = ctx.to_device(rays)
+    R = rays.shape[0]
+    hits = {}
+    for name, sc in (("cur", s), ("tgt", st)):
+        t = ab.LBVHBuilder(False).BuildTree(sc)
+        ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+        c, _ = ab.BVHTreeCollapser().Collapse(t)
+        d_h = ctx.buffer(R * 16)
+        c.trace(d_rays, R, d_h)
+        hits[name] = d_h
+        c.close()
+        t.close()
+    yield {"m": m, "rays": rays, "v32": v32, "u32": u32, "scene": s, "target_scene": st,
+           "d_rays": d_rays, "d_hits": hits["cur"], "d_target": hits["tgt"], "R": R}
+    s.close()
+    st.close()
+
+
+def _gpu_loss_grad(ctx, sp):
+    m, R = sp["m"], sp["R"]
+    V = m["vertices"].shape[0]
+    d_index = ctx.to_device(m["index"].astype(np.uint32).reshape(-1))
+    d_g = ctx.buffer(R * 12)
+    d_loss = ctx.to_device(np.zeros(1, np.float64))
+    d_grad = ctx.to_device(np.zeros((V, 3), np.float32))
+    ctx.depth_loss(sp["d_hits"], sp["d_target"], R, d_g, d_loss)
+    ctx.hit_adjoint(sp["scene"], sp["d_rays"], sp["d_hits"], R, d_g, d_index, V, d_grad)
+    return d_grad.download(np.float32, (V, 3)), float(d_loss.download(np.float64, 1)[0])
+
+
+def test_adjoint_matches_fp64_gradient_and_finite_differences(ctx, sphere):
+    m, rays = sphere["m"], sphere["rays"]
+    R = sphere["R"]
+    hits = sphere["d_hits"].download(np.int32, (R, 4))
+    target = sphere["d_target"].download(np.int32, (R, 4))
+    assert (hits[:, 0] >= 0).mean() > 0.5
+    tt = target_depth(sphere["u32"], m["index"], rays, target[:, 0])
+    g_gpu, loss_gpu = _gpu_loss_grad(ctx, sphere)
+    g64, loss64 = loss_grad_fp64(sphere["v32"], m["index"], rays, hits[:, 0], tt)
+    assert abs(loss_gpu - loss64) <= 1e-4 * loss64
+    e_all = rel_err(g_gpu, g64)
+    ok = well_conditioned_vertices(sphere["v32"], m["index"])
+    front = ok[sphere["v32"][ok, 2] < -0.2]
+    sel = np.random.default_rng(7).choice(front, 1024, replace=False)
+    fd = fd_fixed(sphere["v32"], m["index"], rays, hits[:, 0], tt, sel, h=1e-4, order=4)
+    e_fd = rel_err(g_gpu[sel], fd)
+    fr = fd_retrace(sphere["v32"], m["index"], rays, hits[:, 0], tt, sel[:128], h=1e-4)
+    e_re = rel_err(g_gpu[sel[:128]], fr)
+    print(f"[adjoint] rel err vs FP64 gradient {e_all:.3e}, vs FD {e_fd:.3e}, "
+          f"vs re-traced FD {e_re:.3e}", flush=True)
+    assert e_all < TOL
+    assert e_fd < TOL
+    assert e_re < 5e-2  # kinks where rays change triangle within h
+
+
+def test_generic_tuv_adjoint(ctx, sphere):
+    m, rays, R = sphere["m"], sphere["rays"], sphere["R"]
+    V = m["vertices"].shape[0]
+    hits = sphere["d_hits"].download(np.int32, (R, 4))
+    g_tuv = np.random.default_rng(3).normal(size=(R, 3)).astype(np.float32)
+    d_g = ctx.to_device(g_tuv)
+    d_index = ctx.to_device(m["index"].astype(np.uint32).reshape(-1))
+    d_grad = ctx.to_device(np.zeros((V, 3), np.float32))
+    ctx.hit_adjoint(sphere["scene"], sphere["d_rays"], sphere["d_hits"], R, d_g, d_index, V,
+                    d_grad)
+    g = d_grad.download(np.float32, (V, 3))
+    ref = tuv_adjoint_fp64(sphere["v32"], m["index"], rays, hits[:, 0],
+                           g_tuv.astype(np.float64))
+    assert rel_err(g, ref) < TOL
+    # without an index: one gradient slot per triangle corner
+    N = m["index"].shape[0]
+    d_grad3 = ctx.to_device(np.zeros((3 * N, 3), np.float32))
+    ctx.hit_adjoint(sphere["scene"], sphere["d_rays"], sphere["d_hits"], R, d_g, None, 3 * N,
+                    d_grad3)
+    g3 = d_grad3.download(np.float32, (3 * N, 3))
+    acc = np.zeros((V, 3))
+    np.add.at(acc, m["index"].reshape(-1), g3.astype(np.float64))
+    assert rel_err(acc, ref) < TOL
+
+
+def test_refit_after_vertex_step_matches_rebuild_boxes(ctx, sphere):
+    """One gradient step on the indexed mesh, gathered into the scene's soup,
+    then refit: every box equals a recursive union over the moved vertices."""
+    from helpers import bits
+
+    m, R = sphere["m"], sphere["R"]
+    V = m["vertices"].shape[0]
+    g_gpu, _ = _gpu_loss_grad(ctx, sphere)
+    soup = wl.soup(m["vertices"], m["index"])
+    s = ctx.scene(soup)
+    t = ab.LBVHBuilder(False).BuildTree(s)
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+    d_mesh = ctx.to_device(wl.mesh_float4(m["vertices"]))
+    d_grad = ctx.to_device(g_gpu)
+    d_index = ctx.to_device(m["index"].astype(np.uint32).reshape(-1))
+    ctx.mesh_step(d_mesh, V, d_grad, 0.01, d_index, s)
+    t.refit(s)
+    mesh = d_mesh.download(np.float32, (V, 4))
+    want_mesh = wl.mesh_float4(m["vertices"])
+    want_mesh[:, :3] -= np.float32(0.01) * g_gpu
+    assert np.array_equal(bits(mesh), bits(want_mesh))
+    assert not np.any(d_grad.download(np.float32, (V, 3)))  # cleared for the next step
+    moved = np.zeros((m["index"].size, 4), np.float32)
+    moved[:, :3] = mesh[m["index"].reshape(-1), :3]
+    host = t.download()
+    # recursive union over the refit topology
+    n = m["index"].shape[0]
+    lo = np.zeros((2 * n - 1, 3), np.float32)
+    hi = np.zeros((2 * n - 1, 3), np.float32)
+    tri = moved[:, :3].reshape(n, 3, 3)
+    leaves = np.arange(n - 1, 2 * n - 1)
+    lo[leaves] = tri[host["data"][leaves]].min(axis=1)
+    hi[leaves] = tri[host["data"][leaves]].max(axis=1)
+    order = []
+    stack = [0]
+    while stack:
+        x = stack.pop()
+        if x < n - 1:
+            order.append(x)
+            stack += [host["left"][x], host["right"][x]]
+    for x in reversed(order):
+        lo[x] = np.minimum(lo[host["left"][x]], lo[host["right"][x]])
+        hi[x] = np.maximum(hi[host["left"][x]], hi[host["right"][x]])
+    assert np.array_equal(bits(host["bbmin"][:, :3]), bits(lo))
+    assert np.array_equal(bits(host["bbmax"][:, :3]), bits(hi))
+    t.close()
+    s.close()
+
+
+def test_sah_device_close_to_reference_order(ctx, sphere):
+    t = ab.LBVHBuilder(False).BuildTree(sphere["scene"])
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+    d = ctx.buffer(4)
+    t.sah_device(d)
+    fast = float(d.download(np.float32, 1)[0])
+    exact = t.SAH()
+    assert abs(fast - exact) <= 1e-4 * exact
+    t.close()
