@@ -287,6 +287,50 @@ def trace(c: Collapsed, v4, rays, any_hit=False):
     return res
 
 
+class SbvhStats(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_uint32), ("tri_refs", ctypes.c_uint32),
+                ("duplicates", ctypes.c_uint32), ("depth", ctypes.c_int32),
+                ("sah", ctypes.c_float), ("sah_bvhrt", ctypes.c_float)]
+
+
+class Sbvh:
+    """CPU split-BVH of the reference's CPU path (SplitBVHBuilder) and its
+    multi-threaded closest-hit tracer."""
+
+    def __init__(self, v4, threads=1):
+        L = lib()
+        L.orc_sbvh_build.restype = ctypes.c_void_p
+        L.orc_sbvh_build.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.POINTER(SbvhStats)]
+        L.orc_sbvh_trace.argtypes = [ctypes.c_void_p, _f32p, _f32p, ctypes.c_uint32,
+                                     ctypes.c_int, _i32p]
+        L.orc_sbvh_free.argtypes = [ctypes.c_void_p]
+        self.v4 = _f32(v4)
+        st = SbvhStats()
+        self.h = L.orc_sbvh_build(_p(self.v4, _f32p), self.v4.shape[0] // 3, threads,
+                                  ctypes.byref(st))
+        self.stats = {k: getattr(st, k) for k, _ in SbvhStats._fields_}
+
+    def trace(self, rays, threads=1):
+        rays = _f32(rays)
+        r = rays.shape[0]
+        res = np.zeros((r, 4), np.int32)
+        lib().orc_sbvh_trace(self.h, _p(self.v4, _f32p), _p(rays, _f32p), r, threads,
+                             _p(res, _i32p))
+        return res
+
+    def close(self):
+        if self.h:
+            lib().orc_sbvh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def trace_brute(v4, rays):
     v4 = _f32(v4)
     rays = _f32(rays)

@@ -134,6 +134,23 @@ void orc_ray_inverse(const float* d, float* inv);
 int orc_ray_triangle(const float* v0, const float* v1, const float* v2, const float* o,
                      const float* d, float tmin, float tmax, float* t, float* u, float* v);
 
+/* CPU baseline of the reference's CPU path (oracle/sbvh.cpp): SBVH build
+ * (SplitBVHBuilder.cpp:52-476, "GPU" platform, leaves 1..8, alpha 1e-5) on
+ * `threads` host threads, and a closest-hit tracer over it on `threads`
+ * threads (BVH::traceRecursive semantics).  Results as orc_trace. */
+typedef struct orc_sbvh_stats {
+    uint32_t nodes;
+    uint32_t tri_refs;
+    uint32_t duplicates;
+    int32_t depth;
+    float sah;        /* top-down SAH, BVHNode.cpp:66-81 */
+    float sah_bvhrt;  /* (1.2 inner areas + leaf areas x triangles) / root area */
+} orc_sbvh_stats;
+void* orc_sbvh_build(const float* v4, uint32_t n, int threads, orc_sbvh_stats* stats);
+void orc_sbvh_trace(const void* bvh, const float* v4, const float* rays, uint32_t r, int threads,
+                    int32_t* results);
+void orc_sbvh_free(void* bvh);
+
 #ifdef __cplusplus
 }
 #endif

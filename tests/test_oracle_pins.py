@@ -596,3 +596,24 @@ def test_trbvh_dp_tie_order_closed_form(pattern):
             assert (int(masks[s]) & 0xFF) == want, (trial, s, size, tied)
         assert ties_seen > 0
     assert full == 127
+
+
+# ------------------------------------------------------------------ CPU SBVH baseline
+def test_sbvh_trace_equals_brute_force_and_threads_agree():
+    """The CPU split-BVH (reference CPU path) finds the brute-force closest
+    hits; its multi-threaded build gives the single-threaded tree."""
+    from helpers import camera_rays_host, compare_hits
+
+    v = ab.gen_soup(31, 6000, sigma=0.03)
+    one = oracle.Sbvh(v, threads=1)
+    many = oracle.Sbvh(v, threads=8)
+    assert one.stats == many.stats
+    assert one.stats["tri_refs"] >= 6000 and one.stats["depth"] < 64
+    assert one.stats["sah_bvhrt"] > 0
+    rays = camera_rays_host(48)
+    ref = oracle.trace_brute(v, rays)
+    got1 = one.trace(rays, threads=1)
+    got8 = many.trace(rays, threads=8)
+    assert np.array_equal(got1, got8)
+    compare_hits(got1, ref)
+    assert (ref[:, 0] >= 0).mean() > 0.15
