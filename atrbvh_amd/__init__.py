@@ -140,6 +140,14 @@ def lib():
         L.atr_batch_destroy.argtypes = [_vp]
         L.atr_batch_sah_device.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _vp]
         L.atr_tree_sah_device.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _vp]
+        L.atr_batch_set_config.argtypes = [_vp, ctypes.POINTER(BatchConfig)]
+        L.atr_batch_get_config.argtypes = [_vp, ctypes.POINTER(BatchConfig)]
+        L.atr_batch_trace.argtypes = [_vp, _vp, ctypes.c_uint32, _vp]
+        L.atr_batch_hit_adjoint.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]
+        L.atr_batch_step.argtypes = [_vp, _vp, ctypes.c_float]
+        L.atr_batch_refit.argtypes = [_vp]
+        L.atr_batch_rebuild.argtypes = [_vp, _u32p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_float, ctypes.c_float]
         L.atr_hit_adjoint.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _vp,
                                       ctypes.c_uint32, _vp]
         L.atr_depth_loss.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]
@@ -467,6 +475,11 @@ class Collapsed:
             self.h = None
 
 
+class BatchConfig(ctypes.Structure):
+    """atr_batch_config: launch shape of batched builds."""
+    _fields_ = [("encode_items", ctypes.c_int32), ("sort_items", ctypes.c_int32)]
+
+
 class Batch:
     """Many independent meshes built as one forest in a single launch sequence
     (optionally a captured HIP graph); each mesh's tree equals the one
@@ -495,6 +508,38 @@ class Batch:
         out = np.zeros(self.meshes, np.float32)
         check(lib().atr_batch_sah(self.h, ci, ct, _ptr(out)))
         return out
+
+    def set_config(self, encode_items: int = 16, sort_items: int = 16):
+        cfg = BatchConfig(encode_items, sort_items)
+        check(lib().atr_batch_set_config(self.h, ctypes.byref(cfg)))
+
+    def config(self) -> dict:
+        cfg = BatchConfig()
+        check(lib().atr_batch_get_config(self.h, ctypes.byref(cfg)))
+        return {"encode_items": cfg.encode_items, "sort_items": cfg.sort_items}
+
+    def trace(self, d_rays: DeviceBuffer, rays_per_mesh: int, d_hits: DeviceBuffer):
+        check(lib().atr_batch_trace(self.h, d_rays.ptr, rays_per_mesh, d_hits.ptr))
+
+    def hit_adjoint(self, d_rays: DeviceBuffer, d_hits: DeviceBuffer, n: int,
+                    d_grad_tuv: DeviceBuffer, d_grad: DeviceBuffer):
+        check(lib().atr_batch_hit_adjoint(self.h, d_rays.ptr, d_hits.ptr, n, d_grad_tuv.ptr,
+                                          d_grad.ptr))
+
+    def step(self, d_grad: DeviceBuffer, lr: float):
+        check(lib().atr_batch_step(self.h, d_grad.ptr, lr))
+
+    def refit(self):
+        check(lib().atr_batch_refit(self.h))
+
+    def sah_device(self, d_out: DeviceBuffer, ci: float = 1.2, ct: float = 1.0):
+        check(lib().atr_batch_sah_device(self.h, ci, ct, d_out.ptr))
+
+    def rebuild(self, meshes, treelet: int = 9, iterations: int = 2, ci: float = 1.2,
+                ct: float = 1.0):
+        m = np.ascontiguousarray(meshes, np.uint32)
+        check(lib().atr_batch_rebuild(self.h, _ptr(m, _u32p), m.size, treelet, iterations, ci,
+                                      ct))
 
     def extract(self, mesh: int) -> Tree:
         h = _vp()

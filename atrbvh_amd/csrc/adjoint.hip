@@ -153,6 +153,7 @@ atr_status run_mesh_step(atr_context* ctx, float4* mesh, uint32_t V, float* grad
                                                                                  lr);
         }));
     }
+    if (!index) return ATR_OK;
     const uint64_t m = 3ull * n_tris;
     return launch(ctx, "gather", (double)m * 36.0, [&] {
         k_gather<<<std::max(div_up(m, 256), 1u), 256, 0, ctx->stream>>>(mesh, V, index, m, soup);

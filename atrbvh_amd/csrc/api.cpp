@@ -185,7 +185,8 @@ atr_status atr_sync(atr_context* ctx)
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     uint32_t st = 0;
     ATR_HIP(hipMemcpy(&st, ctx->d_status, 4, hipMemcpyDeviceToHost));
-    if (st != 0) return fail(ATR_ERR_INTERNAL, "a bounded device wait gave up");
+    if (st & 1u) return fail(ATR_ERR_INTERNAL, "a bounded device wait gave up");
+    if (st & 2u) return fail(ATR_ERR_UNSUPPORTED, "a tree was deeper than the traversal stack");
     return ATR_OK;
 }
 
@@ -521,7 +522,7 @@ atr_status atr_trbvh_optimize(atr_tree* t, int treelet_size, int iterations, flo
 {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
     if (treelet_size != 7)
-        return fail(ATR_ERR_INVALID_ARGUMENT, "TRBVH supports treelet size 7 only");
+        return fail(ATR_ERR_UNSUPPORTED, "TRBVH supports treelet size 7 only");
     if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     return treelet_optimize(t, false, treelet_size, iterations, ci, ct);
 }
@@ -955,7 +956,7 @@ static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_
     b->n = n;
     b->h_seg.assign(tri_offsets, tri_offsets + meshes + 1);
     b->passes = forest_passes(meshes);
-    b->hist_blocks = std::max(encode_blocks(n), 1u);
+    b->hist_blocks = encode_forest_blocks(n, 8);  // the most blocks any encode shape uses
     atr_status s = ATR_OK;
     if (s == ATR_OK) s = dalloc_t(ctx, meshes + 1, &b->d_seg);
     if (s == ATR_OK) s = dalloc_t(ctx, 3ull * n, &b->verts);
@@ -1043,11 +1044,12 @@ static atr_status batch_enqueue(atr_batch* b, int treelet, int iterations, float
     if (!b->bbox_given)
         ATR_TRY(run_bbox_forest(ctx, b->verts, b->d_seg, b->meshes, n, b->bbox_acc, b->d_bbox));
     ATR_TRY(run_encode_forest(ctx, b->verts, b->d_seg, b->meshes, n, b->d_bbox, b->keys[0],
-                              b->hist, b->slots, n));
+                              b->hist, b->slots, n, b->encode_items));
     SortWork sw{b->sort_offsets, b->sort_status};
     bool in_alt = false;
     ATR_TRY(run_sort(ctx, 64, b->passes, b->keys[0], b->vals[0], n, b->keys[1], b->vals[1], true,
-                     b->hist, b->hist_blocks, &sw, &in_alt));
+                     b->hist, encode_forest_blocks(n, b->encode_items), &sw, &in_alt,
+                     b->sort_items));
     const int k = in_alt ? 1 : 0;
     ATR_TRY(run_build_forest(ctx, b->keys[k], b->vals[k], b->verts, b->forest, b->d_seg,
                              b->slots));
@@ -1114,6 +1116,27 @@ atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float
     return ATR_OK;
 }
 
+atr_status atr_batch_set_config(atr_batch* b, const atr_batch_config* cfg)
+{
+    if (!b || !cfg) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if ((cfg->encode_items != 8 && cfg->encode_items != 16) ||
+        (cfg->sort_items != 8 && cfg->sort_items != 16))
+        return fail(ATR_ERR_INVALID_ARGUMENT, "encode_items and sort_items must be 8 or 16");
+    if (cfg->encode_items != b->encode_items || cfg->sort_items != b->sort_items)
+        batch_drop_graph(b);
+    b->encode_items = cfg->encode_items;
+    b->sort_items = cfg->sort_items;
+    return ATR_OK;
+}
+
+atr_status atr_batch_get_config(const atr_batch* b, atr_batch_config* cfg)
+{
+    if (!b || !cfg) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    cfg->encode_items = b->encode_items;
+    cfg->sort_items = b->sort_items;
+    return ATR_OK;
+}
+
 atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out)
 {
     if (!b || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -1132,6 +1155,85 @@ atr_status atr_batch_sah_device(atr_batch* b, float ci, float ct, float* d_out)
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     return run_sah_parallel(b->ctx, b->d_seg, b->meshes, 0, (int)b->forest->first_leaf,
                             b->forest->area, ci, ct, d_out);
+}
+
+atr_status atr_batch_trace(atr_batch* b, const float* d_rays, uint32_t rays_per_mesh,
+                           int32_t* d_hits)
+{
+    if (!b || !d_rays || !d_hits) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
+    const uint64_t r = (uint64_t)rays_per_mesh * b->meshes;
+    if (rays_per_mesh == 0) return ATR_OK;
+    if (r >= (1ull << 32)) return fail(ATR_ERR_UNSUPPORTED, "too many rays");
+    return run_trace_forest(b->ctx, b->forest, b->d_seg, b->verts, (const float4*)d_rays,
+                            (uint32_t)r, rays_per_mesh, d_hits);
+}
+
+atr_status atr_batch_hit_adjoint(atr_batch* b, const float* d_rays, const int32_t* d_hits,
+                                 uint32_t num_rays, const float* d_grad_tuv, float* d_grad)
+{
+    if (!b || !d_rays || !d_hits || !d_grad_tuv || !d_grad)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (num_rays == 0) return ATR_OK;
+    return run_hit_adjoint(b->ctx, b->verts, b->n, (const float4*)d_rays, (const int4*)d_hits,
+                           num_rays, d_grad_tuv, nullptr, 3u * b->n, d_grad);
+}
+
+atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr)
+{
+    if (!b || !d_grad) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    return run_mesh_step(b->ctx, b->verts, 3u * b->n, d_grad, lr, nullptr, 0, nullptr);
+}
+
+atr_status atr_batch_refit(atr_batch* b)
+{
+    if (!b) return fail(ATR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
+    return run_refit(b->ctx, b->forest, b->verts, b->counters);
+}
+
+atr_status atr_batch_rebuild(atr_batch* b, const uint32_t* meshes, uint32_t count,
+                             int treelet_size, int iterations, float ci, float ct)
+{
+    if (!b || (!meshes && count)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
+    if (count == 0) return ATR_OK;
+    atr_context* ctx = b->ctx;
+    std::vector<uint32_t> off(count + 1, 0);
+    std::vector<uint8_t> seen(b->meshes, 0);
+    for (uint32_t j = 0; j < count; ++j) {
+        const uint32_t m = meshes[j];
+        if (m >= b->meshes || seen[m])
+            return fail(ATR_ERR_INVALID_ARGUMENT, "mesh indices must be distinct and in range");
+        seen[m] = 1;
+        off[j + 1] = off[j] + (b->h_seg[m + 1] - b->h_seg[m]);
+    }
+    float4* sv = nullptr;
+    uint32_t* d_map = nullptr;
+    atr_batch* sb = nullptr;
+    atr_status s = dalloc_t(ctx, 3ull * off[count], &sv);
+    if (s == ATR_OK) s = dalloc_t(ctx, count, &d_map);
+    for (uint32_t j = 0; j < count && s == ATR_OK; ++j) {
+        const uint32_t m = meshes[j];
+        hipError_t e = hipMemcpyAsync(sv + 3ull * off[j], b->verts + 3ull * b->h_seg[m],
+                                      3ull * (off[j + 1] - off[j]) * sizeof(float4),
+                                      hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "sub-batch vertex copy failed");
+    }
+    if (s == ATR_OK) {
+        hipError_t e = hipMemcpyAsync(d_map, meshes, count * 4ull, hipMemcpyHostToDevice,
+                                      ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "sub-batch map upload failed");
+    }
+    if (s == ATR_OK) s = batch_create(ctx, (const float*)sv, true, off.data(), count, nullptr, &sb);
+    if (s == ATR_OK) s = atr_batch_build(sb, treelet_size, iterations, ci, ct, 0);
+    if (s == ATR_OK)
+        s = run_forest_splice(ctx, sb->forest, sb->d_seg, count, d_map, b->d_seg, b->forest);
+    if (sb) atr_batch_destroy(sb);
+    dfree(ctx, sv);
+    dfree(ctx, d_map);
+    return s;
 }
 
 atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out)

@@ -96,7 +96,67 @@ __global__ __launch_bounds__(256) void k_sah_parallel(const uint32_t* __restrict
     }
 }
 
+__device__ __forceinline__ uint32_t seg_of(const uint32_t* seg, uint32_t meshes, uint32_t i)
+{
+    uint32_t lo = 0, hi = meshes;  // seg[lo] <= i < seg[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Copies the trees of a sub-batch (meshes rebuilt on their own) into the
+// forest slots of the meshes they replace: sub-batch mesh j is forest mesh
+// map[j]; every node, link, triangle index and span moves by the distance
+// between the two meshes' first positions.
+__global__ __launch_bounds__(256) void k_forest_splice(
+    uint32_t sn, uint32_t smeshes, const uint32_t* __restrict__ sseg, int sfirst,
+    const uint32_t* __restrict__ map, const uint32_t* __restrict__ fseg, int ffirst,
+    const int32_t* __restrict__ P, const int32_t* __restrict__ L, const int32_t* __restrict__ R,
+    const int32_t* __restrict__ D, const float4* __restrict__ bmin,
+    const float4* __restrict__ bmax, const float* __restrict__ A, const int2* __restrict__ SP,
+    int32_t* parent, int32_t* left, int32_t* right, int32_t* data, float4* bbmin, float4* bbmax,
+    float* area, int2* span)
+{
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= 2u * sn) return;
+    const bool leaf = (int)g >= sfirst;
+    const uint32_t pos = leaf ? g - (uint32_t)sfirst : g;
+    const uint32_t j = seg_of(sseg, smeshes, pos);
+    const int delta = (int)fseg[map[j]] - (int)sseg[j];
+    auto conv = [&](int y) -> int32_t {
+        if (y < 0) return -1;
+        return y < sfirst ? y + delta : ffirst + (y - sfirst) + delta;
+    };
+    const int dst = leaf ? ffirst + (int)pos + delta : (int)g + delta;
+    parent[dst] = conv(P[g]);
+    left[dst] = conv(L[g]);
+    right[dst] = conv(R[g]);
+    const int d = D[g];
+    data[dst] = d >= 0 ? d + delta : -1;
+    bbmin[dst] = bmin[g];
+    bbmax[dst] = bmax[g];
+    area[dst] = A[g];
+    const int2 sp = SP[g];
+    span[dst] = make_int2(sp.x + delta, sp.y + delta);
+}
+
 }  // namespace
+
+atr_status run_forest_splice(atr_context* ctx, const atr_tree* sub, const uint32_t* d_sseg,
+                             uint32_t smeshes, const uint32_t* d_map, const uint32_t* d_fseg,
+                             atr_tree* f)
+{
+    return launch(ctx, "forest_splice", (double)sub->n * 2.0 * 104.0, [&] {
+        k_forest_splice<<<div_up(2ull * sub->n, 256), 256, 0, ctx->stream>>>(
+            sub->n, smeshes, d_sseg, (int)sub->first_leaf, d_map, d_fseg, (int)f->first_leaf,
+            sub->parent, sub->left, sub->right, sub->data, sub->bbmin, sub->bbmax, sub->area,
+            sub->span, f->parent, f->left, f->right, f->data, f->bbmin, f->bbmax, f->area,
+            f->span);
+    });
+}
 
 atr_status run_sah_parallel(atr_context* ctx, const uint32_t* d_seg, uint32_t meshes,
                             uint32_t n_single, int first_leaf, const float* area, float ci,

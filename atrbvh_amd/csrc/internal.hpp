@@ -149,6 +149,10 @@ struct atr_batch {
     uint32_t* counters = nullptr;
     float* d_sah = nullptr;        // per-mesh SAH
     bool built = false;
+    // launch shape (atr_batch_config): triangles per encode thread, keys per
+    // onesweep thread
+    int encode_items = 16;
+    int sort_items = 16;
     // captured build sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -203,14 +207,16 @@ size_t sort_status_words(uint32_t n, int passes);
 atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint32_t* vals,
                     uint32_t n, void* keys_alt, uint32_t* vals_alt, bool vals_identity,
                     const uint32_t* block_hist, uint32_t hist_blocks, const SortWork* work,
-                    bool* result_in_alt);
+                    bool* result_in_alt, int tile_items = 16);
 // Forests (batched builds): per-mesh boxes, mesh-tagged 30-bit keys, build.
 int forest_passes(uint32_t meshes);
 atr_status run_bbox_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
                            uint32_t meshes, uint32_t n, uint32_t* acc, float* d_bbox);
+uint32_t encode_forest_blocks(uint32_t n, int items);
 atr_status run_encode_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
                              uint32_t meshes, uint32_t n, const float* d_bbox, uint64_t* keys,
-                             uint32_t* block_hist, uint64_t* fill64, uint32_t fill64_count);
+                             uint32_t* block_hist, uint64_t* fill64, uint32_t fill64_count,
+                             int items);
 atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32_t* idx,
                             const float4* verts, atr_tree* t, const uint32_t* d_seg,
                             uint64_t* slots);
@@ -224,6 +230,12 @@ void host_spans(uint32_t n, const int32_t* left, const int32_t* right, int2* spa
 // Batched builds (batch.hip)
 atr_status run_forest_extract(atr_context* ctx, const atr_tree* f, uint32_t s, uint32_t nm,
                               atr_tree* t);
+atr_status run_forest_splice(atr_context* ctx, const atr_tree* sub, const uint32_t* d_sseg,
+                             uint32_t smeshes, const uint32_t* d_map, const uint32_t* d_fseg,
+                             atr_tree* f);
+atr_status run_trace_forest(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,
+                            const float4* verts, const float4* rays, uint32_t r,
+                            uint32_t rays_per_mesh, int32_t* results);
 atr_status run_forest_sah(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,
                           uint32_t meshes, float ci, float ct, float* d_out);
 atr_status run_sah_parallel(atr_context* ctx, const uint32_t* d_seg, uint32_t meshes,

@@ -14,8 +14,6 @@ constexpr int kEncItems = 16;
 constexpr int kEncTile = kEncBlock * kEncItems;  // triangles per encode block
 
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;  // keys per onesweep tile
 constexpr int kSortWaves = kSortBlock / 64;
 
 constexpr uint32_t kFlagAgg = 1u << 30;
@@ -262,6 +260,7 @@ __global__ void k_bbox_forest_final(const uint32_t* acc, uint32_t meshes, float*
 
 // Forest keys: mesh << 30 | 30-bit Morton code of the triangle in its own
 // mesh's box, and the per-block digit histograms of `passes` 8-bit digits.
+template <int ITEMS>
 __global__ __launch_bounds__(kEncBlock) void k_encode_forest(
     const float4* __restrict__ v, const uint32_t* __restrict__ seg, uint32_t meshes, uint32_t n,
     const float* __restrict__ bbox, int passes, uint64_t* __restrict__ keys,
@@ -275,11 +274,11 @@ __global__ __launch_bounds__(kEncBlock) void k_encode_forest(
         hist[w][k - w * passes * 256] = 0;
     }
     __syncthreads();
-    const uint32_t base = blockIdx.x * (uint32_t)kEncTile;
+    const uint32_t base = blockIdx.x * (uint32_t)(kEncBlock * ITEMS);
     const uint32_t first = base + (uint32_t)tid;
     uint32_t m = first < n ? mesh_of(seg, meshes, first) : 0u;
     uint32_t next = seg[m + 1];
-    for (int j = 0; j < kEncItems; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         const uint32_t i = base + (uint32_t)j * kEncBlock + (uint32_t)tid;
         if (i >= n) break;
         while (i >= next) next = seg[++m + 1];
@@ -311,18 +310,38 @@ __global__ __launch_bounds__(kEncBlock) void k_encode_forest(
     }
 }
 
-// Global digit offsets for every pass: offsets[p][d] = number of keys whose
-// digit p is smaller than d.  One block per pass, one thread per digit.
-__global__ __launch_bounds__(256) void k_digit_offsets(const uint32_t* __restrict__ block_hist,
-                                                       uint32_t blocks, int passes,
-                                                       uint32_t* __restrict__ offsets)
+// Global digit counts for every pass: workgroup (c, p) adds the histograms of
+// encode blocks [c*kDigitChunk, (c+1)*kDigitChunk) for pass p, one thread per
+// digit, with all of a thread's loads in flight at once, and adds its sum to
+// totals[p][d] (zeroed beforehand; integer atomics, so the order is free).
+constexpr int kDigitChunk = 16;
+__global__ __launch_bounds__(256) void k_digit_totals(const uint32_t* __restrict__ block_hist,
+                                                      uint32_t blocks, int passes,
+                                                      uint32_t* __restrict__ totals)
+{
+    const int p = blockIdx.y;
+    const int d = threadIdx.x;
+    const uint32_t b0 = blockIdx.x * (uint32_t)kDigitChunk;
+    uint32_t v[kDigitChunk];
+#pragma unroll
+    for (int k = 0; k < kDigitChunk; ++k) {
+        const uint32_t b = b0 + (uint32_t)k;
+        v[k] = b < blocks ? block_hist[((size_t)b * passes + p) * 256 + d] : 0u;
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kDigitChunk; ++k) s += v[k];
+    if (s) atomicAdd(&totals[p * 256 + d], s);
+}
+
+// In place: totals[p][d] -> number of keys whose digit p is smaller than d.
+__global__ __launch_bounds__(256) void k_digit_scan(uint32_t* __restrict__ offsets)
 {
     __shared__ uint32_t tmp[4];
     const int p = blockIdx.x;
     const int d = threadIdx.x;
-    uint32_t s = 0;
-    for (uint32_t b = 0; b < blocks; ++b) s += block_hist[((size_t)b * passes + p) * 256 + d];
-    uint32_t ex = block256_exclusive_sum(s, tmp, nullptr);
+    const uint32_t s = offsets[p * 256 + d];
+    const uint32_t ex = block256_exclusive_sum(s, tmp, nullptr);
     offsets[p * 256 + d] = ex;
 }
 
@@ -358,6 +377,15 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count)
         p[i] = v;
 }
 
+// Zeroes a[0, na) and b[0, nb) in one launch.
+__global__ void k_zero_two(uint32_t* a, uint64_t na, uint32_t* b, uint64_t nb)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < na + nb; i += 256ull * gridDim.x) {
+        if (i < na) a[i] = 0u;
+        else b[i - na] = 0u;
+    }
+}
+
 // ---------------------------------------------------------------- onesweep
 // One stable LSD pass over an 8-bit digit.  Tiles are taken in launch order
 // from an atomic ticket; within a tile each wave ranks its 16 x 64 keys with
@@ -365,7 +393,7 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count)
 // per-digit scan across waves gives tile-local positions, and a decoupled
 // look-back over earlier tiles' published counts gives global positions.
 // Keys are reordered through LDS so writes of equal digits are contiguous.
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ digit_off,
@@ -376,8 +404,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     __shared__ uint32_t s_local[256];
     __shared__ uint32_t s_global[256];
     __shared__ uint32_t s_tmp[4];
-    __shared__ K s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ K s_keys[(kSortBlock * ITEMS)];
+    __shared__ uint32_t s_vals[(kSortBlock * ITEMS)];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -386,14 +414,14 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     for (int k = tid; k < kSortWaves * 256; k += kSortBlock) (&s_wave_hist[0][0])[k] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t tile_base = tile * (uint32_t)kSortTile;
-    const uint32_t wbase = tile_base + (uint32_t)wave * (64u * kSortItems);
+    const uint32_t tile_base = tile * (uint32_t)(kSortBlock * ITEMS);
+    const uint32_t wbase = tile_base + (uint32_t)wave * (64u * ITEMS);
 
-    K key[kSortItems];
-    uint32_t val[kSortItems];
-    uint32_t rank[kSortItems];
+    K key[ITEMS];
+    uint32_t val[ITEMS];
+    uint32_t rank[ITEMS];
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         const uint32_t e = wbase + (uint32_t)j * 64u + (uint32_t)lane;
         if (e < n) {
             key[j] = kin[e];
@@ -405,7 +433,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     }
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         const uint32_t e = wbase + (uint32_t)j * 64u + (uint32_t)lane;
         const bool valid = e < n;
         const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
@@ -466,7 +494,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     __syncthreads();
 
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         const uint32_t e = wbase + (uint32_t)j * 64u + (uint32_t)lane;
         if (e < n) {
             const uint32_t dj = (uint32_t)(key[j] >> shift) & 255u;
@@ -476,7 +504,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
         }
     }
     __syncthreads();
-    const uint32_t valid_count = min((uint32_t)kSortTile, n - tile_base);
+    const uint32_t valid_count = min((uint32_t)(kSortBlock * ITEMS), n - tile_base);
     for (uint32_t k = (uint32_t)tid; k < valid_count; k += kSortBlock) {
         const K kk = s_keys[k];
         const uint32_t dd = (uint32_t)(kk >> shift) & 255u;
@@ -692,7 +720,8 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
 
 // Bottom-up refit over a fixed topology (leaf boxes from the vertices, unions
 // upwards), for trees whose vertices moved.
-__global__ __launch_bounds__(256) void k_refit(uint32_t n, const float4* __restrict__ v,
+__global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
+                                               const float4* __restrict__ v,
                                                const int32_t* __restrict__ parent,
                                                const int32_t* __restrict__ left,
                                                const int32_t* __restrict__ right,
@@ -706,7 +735,7 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, const float4* __restr
     const int blk = (int)blockIdx.x;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    int node = (int)(n - 1 + i);
+    int node = first_leaf + (int)i;
     const uint32_t tri = (uint32_t)data[node];
     const float4 a = v[3ull * tri];
     const float4 b = v[3ull * tri + 1];
@@ -782,9 +811,11 @@ atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const f
     });
 }
 
+// Look-back status words of a sort with the smallest tile (8 items per
+// thread): enough for every tile shape.
 size_t sort_status_words(uint32_t n, int passes)
 {
-    return (size_t)passes * (std::max(div_up(n, kSortTile), 1u) * 256u + 64u);
+    return (size_t)passes * (std::max(div_up(n, (uint64_t)kSortBlock * 8), 1u) * 256u + 64u);
 }
 
 // Stable LSD sort over `passes` 8-bit digits (the low 8*passes key bits).
@@ -796,15 +827,17 @@ size_t sort_status_words(uint32_t n, int passes)
 atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint32_t* vals,
                     uint32_t n, void* keys_alt, uint32_t* vals_alt, bool vals_identity,
                     const uint32_t* block_hist, uint32_t hist_blocks, const SortWork* work,
-                    bool* result_in_alt)
+                    bool* result_in_alt, int tile_items)
 {
     const int kbytes = key_bits == 64 ? 8 : 4;
     if (passes < 1 || passes > kbytes) return fail(ATR_ERR_INTERNAL, "bad sort pass count");
-    const uint32_t tiles = std::max(div_up(n, kSortTile), 1u);
+    if (tile_items != 8 && tile_items != 16)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "sort tile items must be 8 or 16");
+    const uint32_t tiles = std::max(div_up(n, (uint64_t)kSortBlock * tile_items), 1u);
     uint32_t* offsets = work ? work->offsets : nullptr;
     uint32_t* status = work ? work->status : nullptr;
     uint32_t* own_hist = nullptr;
-    const size_t status_words = sort_status_words(n, passes);
+    const size_t status_words = (size_t)passes * (tiles * 256u + 64u);
     if (!work) {
         ATR_TRY(dalloc_t(ctx, (size_t)passes * 256, &offsets));
         ATR_TRY(dalloc_t(ctx, status_words, &status));
@@ -825,12 +858,17 @@ atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint
         }
         block_hist = own_hist;
     }
-    ATR_TRY(launch(ctx, "fill_status", (double)status_words * 4.0, [&] {
-        k_fill_u32<<<std::min<uint32_t>(div_up(status_words, 256), 1024u), 256, 0, ctx->stream>>>(
-            status, 0u, status_words);
+    const uint64_t offset_words = (uint64_t)passes * 256;
+    ATR_TRY(launch(ctx, "fill_status", (double)(status_words + offset_words) * 4.0, [&] {
+        k_zero_two<<<std::min<uint32_t>(div_up(status_words + offset_words, 256), 1024u), 256, 0,
+                     ctx->stream>>>(status, status_words, offsets, offset_words);
     }));
-    ATR_TRY(launch(ctx, "digit_offsets", (double)hist_blocks * passes * 1024.0, [&] {
-        k_digit_offsets<<<passes, 256, 0, ctx->stream>>>(block_hist, hist_blocks, passes, offsets);
+    ATR_TRY(launch(ctx, "digit_totals", (double)hist_blocks * passes * 1024.0, [&] {
+        k_digit_totals<<<dim3(div_up(hist_blocks, kDigitChunk), passes), 256, 0, ctx->stream>>>(
+            block_hist, hist_blocks, passes, offsets);
+    }));
+    ATR_TRY(launch(ctx, "digit_scan", (double)offset_words * 8.0, [&] {
+        k_digit_scan<<<passes, 256, 0, ctx->stream>>>(offsets);
     }));
     void* kbuf[2] = {keys, keys_alt};
     uint32_t* vbuf[2] = {vals, vals_alt};
@@ -843,15 +881,25 @@ atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint
         atr_status s;
         if (key_bits == 64) {
             s = launch(ctx, "onesweep64", bytes, [&] {
-                k_onesweep<uint64_t><<<tiles, kSortBlock, 0, ctx->stream>>>(
-                    (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur], n,
-                    8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                if (tile_items == 16)
+                    k_onesweep<uint64_t, 16><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                        (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur],
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                else
+                    k_onesweep<uint64_t, 8><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                        (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur],
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
             });
         } else {
             s = launch(ctx, "onesweep32", bytes, [&] {
-                k_onesweep<uint32_t><<<tiles, kSortBlock, 0, ctx->stream>>>(
-                    (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur], n,
-                    8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                if (tile_items == 16)
+                    k_onesweep<uint32_t, 16><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                        (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur],
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                else
+                    k_onesweep<uint32_t, 8><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                        (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur],
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
             });
         }
         ATR_TRY(s);
@@ -892,16 +940,27 @@ int forest_passes(uint32_t meshes)
     return (30 + mbits + 7) / 8;
 }
 
+uint32_t encode_forest_blocks(uint32_t n, int items)
+{
+    return std::max(div_up(n, (uint64_t)kEncBlock * items), 1u);
+}
+
 atr_status run_encode_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,
                              uint32_t meshes, uint32_t n, const float* d_bbox, uint64_t* keys,
-                             uint32_t* block_hist, uint64_t* fill64, uint32_t fill64_count)
+                             uint32_t* block_hist, uint64_t* fill64, uint32_t fill64_count,
+                             int items)
 {
+    if (items != 8 && items != 16)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "encode items must be 8 or 16");
     const int passes = forest_passes(meshes);
-    const uint32_t blocks = std::max(encode_blocks(n), 1u);
+    const uint32_t blocks = encode_forest_blocks(n, items);
     return launch(ctx, "encode_forest", (double)n * 56.0, [&] {
-        k_encode_forest<<<blocks, kEncBlock, 0, ctx->stream>>>(verts, d_seg, meshes, n, d_bbox,
-                                                               passes, keys, block_hist, fill64,
-                                                               fill64_count);
+        if (items == 16)
+            k_encode_forest<16><<<blocks, kEncBlock, 0, ctx->stream>>>(
+                verts, d_seg, meshes, n, d_bbox, passes, keys, block_hist, fill64, fill64_count);
+        else
+            k_encode_forest<8><<<blocks, kEncBlock, 0, ctx->stream>>>(
+                verts, d_seg, meshes, n, d_bbox, passes, keys, block_hist, fill64, fill64_count);
     });
 }
 
@@ -943,14 +1002,14 @@ atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32
 atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_t* counters)
 {
     const uint32_t n = t->n;
-    ATR_TRY(launch(ctx, "fill_counters", (double)(n - 1) * 4.0, [&] {
-        k_fill_u32<<<std::min<uint32_t>(div_up(n, 256), 1024u), 256, 0, ctx->stream>>>(
-            counters, 0u, (uint64_t)(n - 1));
+    ATR_TRY(launch(ctx, "fill_counters", (double)t->first_leaf * 4.0, [&] {
+        k_fill_u32<<<std::max(std::min<uint32_t>(div_up(t->first_leaf, 256), 1024u), 1u), 256, 0,
+                     ctx->stream>>>(counters, 0u, (uint64_t)t->first_leaf);
     }));
     return launch(ctx, "refit", (double)n * 176.0, [&] {
-        k_refit<<<div_up(n, 256), 256, 0, ctx->stream>>>(n, verts, t->parent, t->left, t->right,
-                                                         t->data, t->bbmin, t->bbmax, t->area,
-                                                         counters, t->span);
+        k_refit<<<div_up(n, 256), 256, 0, ctx->stream>>>(n, (int)t->first_leaf, verts, t->parent,
+                                                         t->left, t->right, t->data, t->bbmin,
+                                                         t->bbmax, t->area, counters, t->span);
     });
 }
 

@@ -159,6 +159,75 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ rays, 
     results[i] = make_int4(id, __float_as_int(t), __float_as_int(bu), __float_as_int(bv));
 }
 
+// Closest-hit traversal of a batch's binary trees (the forest): ray r belongs
+// to mesh r / rays_per_mesh and starts at that mesh's root (internal slot
+// seg[m]); leaves test their triangle (index into the batch's vertices).
+// Same slab test, child order and Moller-Trumbore as k_trace, so hits equal
+// the per-mesh collapsed traversal's except for exact ties.  A tree deeper
+// than the stack stops the ray and raises bit 1 of `err`.
+constexpr int kForestStack = 128;
+__global__ __launch_bounds__(256) void k_trace_forest(
+    const float4* __restrict__ rays, uint32_t R, uint32_t rays_per_mesh,
+    const uint32_t* __restrict__ seg, int first_leaf, const int32_t* __restrict__ left,
+    const int32_t* __restrict__ right, const int32_t* __restrict__ data,
+    const float4* __restrict__ bbmin, const float4* __restrict__ bbmax,
+    const float4* __restrict__ v, int4* __restrict__ results, uint32_t* err)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= R) return;
+    const float4 r0 = rays[2ull * i];
+    const float4 r1 = rays[2ull * i + 1];
+    const float ox = r0.x, oy = r0.y, oz = r0.z, tmin = r0.w;
+    const float dx = r1.x, dy = r1.y, dz = r1.z;
+    float t = r1.w;
+    const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+    int id = -1;
+    float bu = 0.0f, bv = 0.0f;
+    int stack[kForestStack];
+    int sp = 0;
+    int node = t > tmin ? (int)seg[i / rays_per_mesh] : -1;
+    while (node >= 0) {
+        if (node >= first_leaf) {
+            const int tri = data[node];
+            const float4 a = v[3ull * tri], b = v[3ull * tri + 1], c = v[3ull * tri + 2];
+            const float4 e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+            const float4 e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+            float tt, uu, vv;
+            if (intersect(a, e1, e2, ox, oy, oz, dx, dy, dz, tmin, t, tt, uu, vv)) {
+                t = tt;
+                bu = uu;
+                bv = vv;
+                id = tri;
+            }
+            node = sp > 0 ? stack[--sp] : -1;
+            continue;
+        }
+        const int c0 = left[node], c1 = right[node];
+        const float4 a0 = bbmin[c0], a1 = bbmax[c0], b0 = bbmin[c1], b1 = bbmax[c1];
+        float t0, t1;
+        const bool h0 = slab(a0.x, a1.x, a0.y, a1.y, a0.z, a1.z, ox, oy, oz, ix, iy, iz, tmin, t,
+                             t0);
+        const bool h1 = slab(b0.x, b1.x, b0.y, b1.y, b0.z, b1.z, ox, oy, oz, ix, iy, iz, tmin, t,
+                             t1);
+        if (h0 && h1) {
+            const bool swap = t0 > t1;
+            if (sp == kForestStack) {
+                atomicOr(err, 2u);
+                break;
+            }
+            stack[sp++] = swap ? c0 : c1;
+            node = swap ? c1 : c0;
+        } else if (h0) {
+            node = c0;
+        } else if (h1) {
+            node = c1;
+        } else {
+            node = sp > 0 ? stack[--sp] : -1;
+        }
+    }
+    results[i] = make_int4(id, __float_as_int(t), __float_as_int(bu), __float_as_int(bv));
+}
+
 // ------------------------------------------------------------------ ray gen
 __global__ __launch_bounds__(256) void k_rays_pinhole(float ex, float ey, float ez, float rx,
                                                       float ry, float rz, float ux, float uy,
@@ -284,6 +353,18 @@ __global__ __launch_bounds__(256) void k_rays_diffuse(const float4* __restrict__
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
+
+atr_status run_trace_forest(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,
+                            const float4* verts, const float4* rays, uint32_t r,
+                            uint32_t rays_per_mesh, int32_t* results)
+{
+    if (r == 0) return ATR_OK;
+    return launch(ctx, "trace_forest", (double)r * 48.0, [&] {
+        k_trace_forest<<<div_up(r, 256), 256, 0, ctx->stream>>>(
+            rays, r, rays_per_mesh, d_seg, (int)f->first_leaf, f->left, f->right, f->data,
+            f->bbmin, f->bbmax, verts, (int4*)results, ctx->d_status);
+    });
+}
 
 atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uint32_t r,
                      int any_hit, int32_t* results, int depth)

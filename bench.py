@@ -93,6 +93,45 @@ def cpu_baseline(sample_tris: int, min_seconds: float = 10.0):
                       f"({sum(runs):.1f} s total)"}
 
 
+def cpu_paths(v: np.ndarray, rays: np.ndarray | None, gpu_hits: np.ndarray | None):
+    """The reference's CPU path restated (CPU split-BVH builder and closest-hit
+    tracer, oracle/sbvh.cpp) on the host cores: the SBVH build of a
+    262,144-triangle prefix of the scene on one thread (the reference builder
+    is single-threaded), the build of the whole scene and the trace of the
+    primary rays on T threads (T = host CPUs, at most 16)."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = {"threads": threads}
+    sample = 262144
+    t0 = time.perf_counter()
+    one = oracle.Sbvh(v[: 3 * sample], threads=1)
+    dt = time.perf_counter() - t0
+    out["sbvh_build_1t"] = {"triangles": sample, "s": round(dt, 3),
+                            "Mtris_s": round(sample / dt / 1e6, 4),
+                            "sah_bvhrt": round(one.stats["sah_bvhrt"], 4)}
+    one.close()
+    t0 = time.perf_counter()
+    full = oracle.Sbvh(v, threads=threads)
+    dt = time.perf_counter() - t0
+    out["sbvh_build_mt"] = {"triangles": v.shape[0] // 3, "s": round(dt, 3),
+                            "Mtris_s": round(v.shape[0] / 3 / dt / 1e6, 4),
+                            "sah": round(full.stats["sah"], 4),
+                            "sah_bvhrt": round(full.stats["sah_bvhrt"], 4),
+                            "duplicates": full.stats["duplicates"]}
+    if rays is not None:
+        t0 = time.perf_counter()
+        hits = full.trace(rays, threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_trace_mt"] = {"rays": rays.shape[0], "s": round(dt, 3),
+                               "Mrays_s": round(rays.shape[0] / dt / 1e6, 3)}
+        if gpu_hits is not None:
+            out["cpu_trace_mt"]["same_hit_as_gpu"] = round(
+                float(np.mean(hits[:, 0] == gpu_hits[:, 0])), 6)
+    full.close()
+    return out
+
+
 def batched_leg(ctx, meshes: int, reps: int):
     """Batched builds (4,096 soups of 64..65,536 triangles, ~39M triangles,
     per-mesh seeds 3000 + m, ATRBVH 9/2, 30-bit keys) as one captured launch
@@ -102,7 +141,10 @@ def batched_leg(ctx, meshes: int, reps: int):
     total = int(off[-1])
     b = ab.Batch(ctx, v, off)
     del v
-    b.build(9, 2, graph=True)  # capture + first replay
+    from atrbvh_amd.autotune import tune_launch
+
+    tuned = tune_launch(ctx, b, 9, 2, reps=2)
+    b.build(9, 2, graph=True)  # capture + first replay with the chosen shape
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -148,10 +190,11 @@ def batched_leg(ctx, meshes: int, reps: int):
             "plain_builds_per_s": round(meshes / (loop_ms * 1e-3), 1),
             "speedup_vs_plain": round(loop_ms / graph_ms, 2),
             "sah_identical": bool(same),
-            "mean_sah": round(float(np.mean(sah_batch)), 4)}
+            "mean_sah": round(float(np.mean(sah_batch)), 4),
+            "autotune": tuned}
 
 
-def differentiable_leg(ctx, steps: int = 50, lr: float = 0.02):
+def differentiable_leg(ctx, steps: int = 50, lr: float = 0.005):
     """Refit-driven optimisation of the 262,144-triangle perturbed sphere
     toward the unit sphere's depths under 1024 x 1024 orthographic rays: per
     step trace -> depth loss -> hit adjoint (atomic scatter-add over the fixed
@@ -224,6 +267,100 @@ def differentiable_leg(ctx, steps: int = 50, lr: float = 0.02):
             "sah_drift": round(sah_refit / sah_fresh - 1.0, 5)}
 
 
+def batch_ortho_rays(v: np.ndarray, off: np.ndarray, per_side: int) -> np.ndarray:
+    """per_side^2 orthographic rays along +z over every mesh's box, mesh-major."""
+    meshes = off.size - 1
+    out = np.zeros((meshes * per_side * per_side, 8), np.float32)
+    c = (np.arange(per_side, dtype=np.float64) + 0.5) / per_side
+    for m in range(meshes):
+        vm = v[3 * int(off[m]): 3 * int(off[m + 1]), :3]
+        lo, hi = vm.min(axis=0).astype(np.float64), vm.max(axis=0).astype(np.float64)
+        y, x = np.meshgrid(lo[1] + c * (hi[1] - lo[1]), lo[0] + c * (hi[0] - lo[0]),
+                           indexing="ij")
+        r = out[m * per_side * per_side:(m + 1) * per_side * per_side]
+        r[:, 0] = x.reshape(-1)
+        r[:, 1] = y.reshape(-1)
+        r[:, 2] = lo[2] - 0.1
+        r[:, 6] = 1.0
+        r[:, 7] = np.inf
+    return out
+
+
+def batched_differentiable_leg(ctx, meshes: int = 64, steps: int = 100, lr: float = 0.002,
+                               per_side: int = 512):
+    """64 soups of 1,024..524,288 triangles (per-mesh seeds 5000 + m), vertices
+    perturbed by N(0, 0.002); every step: batched trace (262,144 orthographic
+    rays per mesh) -> depth loss against the unperturbed meshes -> batched hit
+    adjoint -> soup step -> batched refit -> per-mesh SAH, with an ATRBVH
+    rebuild of every mesh whose refit SAH exceeds 1.10x its post-build SAH.
+    Compared with rebuilding every mesh every step."""
+    off = ab.gen_batch_sizes(5, meshes, 1024, 524288)
+    v0 = ab.gen_soup_batch(5000, off)
+    vp = v0.copy()
+    vp[:, :3] += np.random.default_rng(5).normal(0.0, 0.002, (vp.shape[0], 3)).astype(np.float32)
+    rays = batch_ortho_rays(v0, off, per_side)
+    per = per_side * per_side
+    R = rays.shape[0]
+    n = int(off[-1])
+    d_rays = ctx.to_device(rays)
+    del rays
+    d_target = ctx.buffer(R * 16)
+    d_hits = ctx.buffer(R * 16)
+    d_g = ctx.buffer(R * 12)
+    d_grad = ctx.to_device(np.zeros((3 * n, 3), np.float32))
+    d_loss = ctx.to_device(np.zeros(steps, np.float64))
+    d_sah = ctx.buffer(4 * meshes)
+    bt = ab.Batch(ctx, v0, off)
+    bt.build(9, 2, graph=False)
+    bt.trace(d_rays, per, d_target)
+    ctx.sync()
+    bt.close()
+    results = {}
+    for mode in ("refit", "rebuild_every_step"):
+        b = ab.Batch(ctx, vp, off)
+        b.build(9, 2, graph=True)
+        b.sah_device(d_sah)
+        sah_built = d_sah.download(np.float32, meshes).copy()
+        rebuilds = 0
+        d_loss.upload(np.zeros(steps, np.float64))
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            b.trace(d_rays, per, d_hits)
+            ctx.depth_loss(d_hits, d_target, R, d_g, d_loss, offset=k)
+            b.hit_adjoint(d_rays, d_hits, R, d_g, d_grad)
+            b.step(d_grad, lr)
+            if mode == "refit":
+                b.refit()
+                b.sah_device(d_sah)
+                sah = d_sah.download(np.float32, meshes)
+                flagged = np.nonzero(sah > np.float32(1.10) * sah_built)[0]
+                if flagged.size:
+                    b.rebuild(flagged, 9, 2)
+                    b.sah_device(d_sah)
+                    sah_built[flagged] = d_sah.download(np.float32, meshes)[flagged]
+                    rebuilds += int(flagged.size)
+            else:
+                b.build(9, 2, graph=True)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        b.sah_device(d_sah)
+        final_sah = d_sah.download(np.float32, meshes).copy()
+        losses = d_loss.download(np.float64, steps)
+        b.close()
+        results[mode] = {"ms_per_step": round(dt * 1000.0 / steps, 3),
+                         "steps_per_s": round(steps / dt, 2), "rebuilds": rebuilds,
+                         "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                         "final_sah": final_sah}
+    drift = results["refit"]["final_sah"] / results["rebuild_every_step"]["final_sah"] - 1.0
+    for r in results.values():
+        r["mean_final_sah"] = round(float(np.mean(r.pop("final_sah"))), 4)
+    return {"meshes": meshes, "triangles": n, "rays": R, "steps": steps,
+            "refit": results["refit"], "rebuild_every_step": results["rebuild_every_step"],
+            "sah_drift_mean": round(float(np.mean(drift)), 5),
+            "sah_drift_max": round(float(np.max(drift)), 5)}
+
+
 def copy_calibration(nbytes: int = 256 << 20):
     import torch
 
@@ -246,6 +383,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-build leg")
     ap.add_argument("--batch-meshes", type=int, default=4096)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable leg")
+    ap.add_argument("--no-batch-adjoint", action="store_true",
+                    help="skip the batched differentiable leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -351,6 +490,7 @@ def main():
 
     # ---------------------------------------------------------------- traversal
     trace = None
+    prim_rays_host = prim_hits_host = None
     if not args.no_trace:
         r = GRID * GRID
         d_prim = ctx.buffer(r * 32)
@@ -362,6 +502,8 @@ def main():
         ctx.rays_diffuse(scene, d_prim, d_res, r, 7, float("inf"), d_sec)
         ctx.sync()
         res = d_res.download(np.int32, (r, 4))
+        prim_rays_host = d_prim.download(np.float32, (r, 8))
+        prim_hits_host = res
         n_diffuse = int((res[:, 0] >= 0).sum())
         treps = 5
         ctx.sync()
@@ -401,10 +543,22 @@ def main():
         log(f"[bench] adjoint loop {adjoint['ms_per_step']} ms/step, loss "
             f"{adjoint['loss_first']} -> {adjoint['loss_last']}, rebuilds {adjoint['rebuilds']}")
 
+    batch_adjoint = None
+    if not args.no_batch_adjoint:
+        log("[bench] batched differentiable leg: 64 meshes, 100 steps")
+        batch_adjoint = batched_differentiable_leg(ctx)
+        log(f"[bench] batched adjoint: refit path {batch_adjoint['refit']['ms_per_step']} ms/step "
+            f"({batch_adjoint['refit']['rebuilds']} rebuilds), rebuild-every-step "
+            f"{batch_adjoint['rebuild_every_step']['ms_per_step']} ms/step, SAH drift "
+            f"{batch_adjoint['sah_drift_mean']}")
+
     cpu = None
+    cpu_ref = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline on {args.cpu_tris:,} triangles")
         cpu = cpu_baseline(args.cpu_tris)
+        log("[bench] CPU SBVH builder / tracer")
+        cpu_ref = cpu_paths(v, prim_rays_host, prim_hits_host)
 
     out = {
         "metric": METRIC,
@@ -432,10 +586,12 @@ def main():
         "trace": trace,
         "batched": batched,
         "differentiable": adjoint,
+        "batched_differentiable": batch_adjoint,
         "gpu_ms_per_step": round(gpu_ms, 4),
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_reference_path": cpu_ref,
     }
     print(json.dumps(out), flush=True)
     scene.close()

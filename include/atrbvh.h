@@ -272,12 +272,44 @@ atr_status atr_batch_vertices(atr_batch* b, float** d_vertices);
  * parameter set) and replayed; it allocates nothing after the first call. */
 atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float ci, float ct,
                            int use_graph);
+/* Launch shape of batched builds (results do not depend on it): triangles per
+ * thread of the encode kernel and keys per thread of a radix-sort tile, 8 or
+ * 16 each (default 16/16).  Changing it drops the captured graph.  Chosen per
+ * workload by atrbvh_amd.autotune. */
+typedef struct atr_batch_config {
+    int32_t encode_items;
+    int32_t sort_items;
+} atr_batch_config;
+atr_status atr_batch_set_config(atr_batch* b, const atr_batch_config* cfg);
+atr_status atr_batch_get_config(const atr_batch* b, atr_batch_config* cfg);
 /* SoABVHTree::SAH of every mesh (same summation order), into host out[num_meshes].
  * Blocking. */
 atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out);
 /* Per-mesh SAH into device memory d_out[num_meshes], asynchronously, with
  * the double-precision parallel reduction of atr_tree_sah_device. */
 atr_status atr_batch_sah_device(atr_batch* b, float ci, float ct, float* d_out);
+/* Closest-hit traversal of every mesh on its binary tree: ray r (d_rays
+ * float4[2R], R = rays_per_mesh * num_meshes) belongs to mesh
+ * r / rays_per_mesh; d_hits int32[4R] as atr_trace writes them, with
+ * triangle ids indexing the batch's vertex array.  Hits equal the per-mesh
+ * collapsed traversal's except exact ties. */
+atr_status atr_batch_trace(atr_batch* b, const float* d_rays, uint32_t rays_per_mesh,
+                           int32_t* d_hits);
+/* atr_hit_adjoint over the batch's triangle soup: d_grad is float[9T] (one
+ * slot per triangle corner, accumulated). */
+atr_status atr_batch_hit_adjoint(atr_batch* b, const float* d_rays, const int32_t* d_hits,
+                                 uint32_t num_rays, const float* d_grad_tuv, float* d_grad);
+/* Vertex step on the batch's soup: vertices -= lr * d_grad (float[9T], then
+ * cleared). */
+atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr);
+/* Bottom-up refit of every mesh's tree over the batch's current vertices. */
+atr_status atr_batch_refit(atr_batch* b);
+/* Rebuilds (LBVH 30-bit + ATRBVH) only the listed meshes from their current
+ * vertices and puts their new trees in place; the other meshes keep theirs.
+ * meshes: host uint32[count], distinct indices.  Allocates a temporary
+ * sub-batch. */
+atr_status atr_batch_rebuild(atr_batch* b, const uint32_t* meshes, uint32_t count,
+                             int treelet_size, int iterations, float ci, float ct);
 /* Mesh `mesh` as a standalone tree in the BVHRT layout (caller destroys it). */
 atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out);
 atr_status atr_batch_destroy(atr_batch* b);
