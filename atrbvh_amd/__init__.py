@@ -144,14 +144,15 @@ def lib():
         L.atr_batch_get_config.argtypes = [_vp, ctypes.POINTER(BatchConfig)]
         L.atr_batch_trace.argtypes = [_vp, _vp, ctypes.c_uint32, _vp]
         L.atr_batch_hit_adjoint.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]
-        L.atr_batch_step.argtypes = [_vp, _vp, ctypes.c_float]
+        L.atr_batch_step.argtypes = [_vp, _vp, ctypes.c_float, ctypes.c_float]
         L.atr_batch_refit.argtypes = [_vp]
         L.atr_batch_rebuild.argtypes = [_vp, _u32p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_float]
         L.atr_hit_adjoint.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _vp,
                                       ctypes.c_uint32, _vp]
-        L.atr_depth_loss.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]
-        L.atr_mesh_step.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_float, _vp, _vp]
+        L.atr_depth_loss.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp, _vp]
+        L.atr_mesh_step.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_float,
+                                    ctypes.c_float, _vp, _vp]
         _lib = L
     return _lib
 
@@ -308,15 +309,16 @@ class Context:
                                     d_grad.ptr))
 
     def depth_loss(self, d_hits: DeviceBuffer, d_target: DeviceBuffer, n: int,
-                   d_grad_tuv: DeviceBuffer, d_loss: DeviceBuffer, offset: int = 0):
-        check(lib().atr_depth_loss(self.h, d_hits.ptr, d_target.ptr, n, d_grad_tuv.ptr,
-                                   d_loss.ptr + 8 * offset))
+                   d_grad_tuv: DeviceBuffer, d_loss: DeviceBuffer, offset: int = 0,
+                   same_triangle: bool = False):
+        check(lib().atr_depth_loss(self.h, d_hits.ptr, d_target.ptr, n, int(same_triangle),
+                                   d_grad_tuv.ptr, d_loss.ptr + 8 * offset))
 
     def mesh_step(self, d_mesh: DeviceBuffer, n_vertices: int, d_grad: DeviceBuffer | None,
-                  lr: float, d_index: DeviceBuffer, scene: "Scene"):
+                  lr: float, d_index: DeviceBuffer, scene: "Scene", max_step: float = 0.0):
         check(lib().atr_mesh_step(self.h, d_mesh.ptr, n_vertices,
-                                  d_grad.ptr if d_grad is not None else None, lr, d_index.ptr,
-                                  scene.h))
+                                  d_grad.ptr if d_grad is not None else None, lr, max_step,
+                                  d_index.ptr, scene.h))
 
     def rays_pinhole(self, eye, basis, tan_half: float, w: int, h: int, tmax: float,
                      d_rays: DeviceBuffer):
@@ -526,8 +528,8 @@ class Batch:
         check(lib().atr_batch_hit_adjoint(self.h, d_rays.ptr, d_hits.ptr, n, d_grad_tuv.ptr,
                                           d_grad.ptr))
 
-    def step(self, d_grad: DeviceBuffer, lr: float):
-        check(lib().atr_batch_step(self.h, d_grad.ptr, lr))
+    def step(self, d_grad: DeviceBuffer, lr: float, max_step: float = 0.0):
+        check(lib().atr_batch_step(self.h, d_grad.ptr, lr, max_step))
 
     def refit(self):
         check(lib().atr_batch_refit(self.h))

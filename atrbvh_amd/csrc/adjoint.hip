@@ -67,11 +67,13 @@ __global__ __launch_bounds__(256) void k_hit_adjoint(
     }
 }
 
-// Loss sum (t - t_target)^2 over rays that hit in both traces; per-ray
-// upstream gradient (2 (t - t_target), 0, 0), zero elsewhere.
+// Loss sum (t - t_target)^2 over rays that hit in both traces (and, when
+// same_tri, hit the same triangle in both); per-ray upstream gradient
+// (2 (t - t_target), 0, 0), zero elsewhere.
 __global__ __launch_bounds__(256) void k_depth_loss(const int4* __restrict__ hits,
                                                     const int4* __restrict__ target, uint32_t R,
-                                                    float* __restrict__ g_tuv, double* loss)
+                                                    int same_tri, float* __restrict__ g_tuv,
+                                                    double* loss)
 {
     __shared__ double s_part[4];
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(256) void k_depth_loss(const int4* __restrict__ hit
         const int4 h = hits[r];
         const int4 tg = target[r];
         float g = 0.0f;
-        if (h.x >= 0 && tg.x >= 0) {
+        if (h.x >= 0 && tg.x >= 0 && (!same_tri || h.x == tg.x)) {
             const float d = __int_as_float(h.y) - __int_as_float(tg.y);
             g = 2.0f * d;
             l = (double)d * (double)d;
@@ -95,17 +97,31 @@ __global__ __launch_bounds__(256) void k_depth_loss(const int4* __restrict__ hit
     if (threadIdx.x == 0) unsafeAtomicAdd(loss, (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]));
 }
 
-// Gradient step on indexed mesh vertices (the gradient is cleared for the
-// next accumulation), then the triangle soup is gathered from the mesh.
+// Gradient step on indexed mesh vertices: each vertex moves by -lr * grad,
+// shortened to length max_step when it is longer (max_step <= 0: no limit);
+// the gradient is cleared for the next accumulation.  Then the triangle soup
+// is gathered from the mesh.
 __global__ __launch_bounds__(256) void k_vertex_step(float4* mesh, uint32_t V, float* grad,
-                                                     float lr)
+                                                     float lr, float max_step)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= V) return;
     float4 p = mesh[i];
-    p.x -= lr * grad[3ull * i];
-    p.y -= lr * grad[3ull * i + 1];
-    p.z -= lr * grad[3ull * i + 2];
+    float sx = lr * grad[3ull * i];
+    float sy = lr * grad[3ull * i + 1];
+    float sz = lr * grad[3ull * i + 2];
+    if (max_step > 0.0f) {
+        const float len = sqrtf(sx * sx + sy * sy + sz * sz);
+        if (len > max_step) {
+            const float k = max_step / len;
+            sx *= k;
+            sy *= k;
+            sz *= k;
+        }
+    }
+    p.x -= sx;
+    p.y -= sy;
+    p.z -= sz;
     mesh[i] = p;
     grad[3ull * i] = 0.0f;
     grad[3ull * i + 1] = 0.0f;
@@ -136,21 +152,21 @@ atr_status run_hit_adjoint(atr_context* ctx, const float4* v, uint32_t n_tris,
 }
 
 atr_status run_depth_loss(atr_context* ctx, const int4* hits, const int4* target, uint32_t R,
-                          float* g_tuv, double* loss)
+                          int same_tri, float* g_tuv, double* loss)
 {
     return launch(ctx, "depth_loss", (double)R * 44.0, [&] {
         k_depth_loss<<<std::max(div_up(R, 256), 1u), 256, 0, ctx->stream>>>(hits, target, R,
-                                                                            g_tuv, loss);
+                                                                            same_tri, g_tuv, loss);
     });
 }
 
 atr_status run_mesh_step(atr_context* ctx, float4* mesh, uint32_t V, float* grad, float lr,
-                         const uint32_t* index, uint32_t n_tris, float4* soup)
+                         float max_step, const uint32_t* index, uint32_t n_tris, float4* soup)
 {
     if (grad) {
         ATR_TRY(launch(ctx, "vertex_step", (double)V * 40.0, [&] {
             k_vertex_step<<<std::max(div_up(V, 256), 1u), 256, 0, ctx->stream>>>(mesh, V, grad,
-                                                                                 lr);
+                                                                                 lr, max_step);
         }));
     }
     if (!index) return ATR_OK;

@@ -485,12 +485,17 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int itera
     int32_t* tris = nullptr;
     float* nsah = nullptr;
     uint32_t* counters = nullptr;
+    int4* rec = nullptr;
     int32_t* d_sched = nullptr;
     int rounds = 0;
     atr_status s = ATR_OK;
     if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n - 1, &tris);
     if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n - 1, &nsah);
     if (s == ATR_OK) s = dalloc_t(ctx, n, &counters);
+    if (s == ATR_OK && agg && iterations > 0) {
+        s = dalloc_t(ctx, t->first_leaf, &rec);
+        if (s == ATR_OK) s = run_make_records(ctx, t, rec);
+    }
     if (s == ATR_OK && !agg) {
         std::vector<int32_t> sched;
         rounds = trbvh_schedule(treelet, 32, sched);
@@ -506,7 +511,7 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int itera
     int gamma = treelet;
     for (int it = 0; it < iterations && s == ATR_OK; ++it) {
         if (agg)
-            s = run_atrbvh_pass(ctx, t, treelet, gamma, ci, ct, tris, nsah, counters);
+            s = run_atrbvh_pass(ctx, t, treelet, gamma, ci, ct, tris, nsah, counters, rec);
         else
             s = run_trbvh_pass(ctx, t, gamma, ci, ct, tris, nsah, counters, d_sched, rounds);
         gamma *= 2;
@@ -514,6 +519,7 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int itera
     dfree(ctx, tris);
     dfree(ctx, nsah);
     dfree(ctx, counters);
+    dfree(ctx, rec);
     dfree(ctx, d_sched);
     return s;
 }
@@ -902,24 +908,26 @@ atr_status atr_hit_adjoint(atr_context* ctx, atr_scene* scene, const float* d_ra
 }
 
 atr_status atr_depth_loss(atr_context* ctx, const int32_t* d_hits, const int32_t* d_target_hits,
-                          uint32_t num_rays, float* d_grad_tuv, double* d_loss)
+                          uint32_t num_rays, int same_triangle, float* d_grad_tuv,
+                          double* d_loss)
 {
     if (!ctx || !d_hits || !d_target_hits || !d_grad_tuv || !d_loss)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (num_rays == 0) return ATR_OK;
     return run_depth_loss(ctx, (const int4*)d_hits, (const int4*)d_target_hits, num_rays,
-                          d_grad_tuv, d_loss);
+                          same_triangle ? 1 : 0, d_grad_tuv, d_loss);
 }
 
 atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_vertices,
-                         float* d_grad, float lr, const uint32_t* d_index, atr_scene* scene)
+                         float* d_grad, float lr, float max_step, const uint32_t* d_index,
+                         atr_scene* scene)
 {
     if (!ctx || !d_mesh_vertices || !d_index || !scene)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (scene->ctx != ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "scene of another context");
     if (num_vertices == 0) return fail(ATR_ERR_INVALID_ARGUMENT, "empty mesh");
-    return run_mesh_step(ctx, (float4*)d_mesh_vertices, num_vertices, d_grad, lr, d_index,
-                         scene->n, scene->verts);
+    return run_mesh_step(ctx, (float4*)d_mesh_vertices, num_vertices, d_grad, lr, max_step,
+                         d_index, scene->n, scene->verts);
 }
 
 atr_status atr_tree_sah_device(atr_tree* t, float ci, float ct, float* d_out)
@@ -973,6 +981,7 @@ static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_
     if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &b->tris);
     if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &b->nsah);
     if (s == ATR_OK) s = dalloc_t(ctx, n, &b->counters);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->rec);
     if (s == ATR_OK) s = dalloc_t(ctx, meshes, &b->d_sah);
     if (s == ATR_OK) {
         atr_tree* f = new atr_tree();
@@ -1054,9 +1063,10 @@ static atr_status batch_enqueue(atr_batch* b, int treelet, int iterations, float
     ATR_TRY(run_build_forest(ctx, b->keys[k], b->vals[k], b->verts, b->forest, b->d_seg,
                              b->slots));
     int gamma = treelet;
+    if (iterations > 0) ATR_TRY(run_make_records(ctx, b->forest, b->rec));
     for (int it = 0; it < iterations; ++it) {
         ATR_TRY(run_atrbvh_pass(ctx, b->forest, treelet, gamma, ci, ct, b->tris, b->nsah,
-                                b->counters));
+                                b->counters, b->rec));
         gamma *= 2;
     }
     return ATR_OK;
@@ -1179,10 +1189,10 @@ atr_status atr_batch_hit_adjoint(atr_batch* b, const float* d_rays, const int32_
                            num_rays, d_grad_tuv, nullptr, 3u * b->n, d_grad);
 }
 
-atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr)
+atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr, float max_step)
 {
     if (!b || !d_grad) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
-    return run_mesh_step(b->ctx, b->verts, 3u * b->n, d_grad, lr, nullptr, 0, nullptr);
+    return run_mesh_step(b->ctx, b->verts, 3u * b->n, d_grad, lr, max_step, nullptr, 0, nullptr);
 }
 
 atr_status atr_batch_refit(atr_batch* b)
@@ -1276,6 +1286,7 @@ atr_status atr_batch_destroy(atr_batch* b)
     dfree(ctx, b->tris);
     dfree(ctx, b->nsah);
     dfree(ctx, b->counters);
+    dfree(ctx, b->rec);
     dfree(ctx, b->d_sah);
     delete b;
     return ATR_OK;

@@ -93,15 +93,24 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // ------------------------------------------------------------------ cost
 // copyTreeKernel + calculateCostKernel (TreeCollapser.cu:13-93) in one
 // bottom-up pass.  flags bit0 = collapsible, bit1 = "a climb leaves this
-// node" (leaf, or both children deliver and the node is collapsible).
+// node" (leaf, or both children deliver and the node is collapsible).  What a
+// parent needs from a child -- triangle count, cost, flags, height -- is one
+// 16-byte record, stored device-coherently only when the parent's hand-over
+// crosses workgroups (the other child's thread may then run on another XCD).
+__device__ __forceinline__ void put_state(int4* st, int node, int4 v, bool coherent)
+{
+    if (coherent) st_coh(&st[node], v);
+    else st[node] = v;
+}
+
 __global__ __launch_bounds__(256) void k_collapse_cost(
     uint32_t N, float ci, float ct, const int32_t* __restrict__ parent,
     const int32_t* __restrict__ left, const int32_t* __restrict__ right,
     const int32_t* __restrict__ data, const float4* __restrict__ bbmin,
     const float4* __restrict__ bbmax, const float* __restrict__ tarea, int32_t* c_parent,
     int32_t* c_left, int32_t* c_right, int32_t* c_data, int32_t* c_count, float4* c_bbmin,
-    float4* c_bbmax, float* cost, uint32_t* flags, int32_t* height, uint32_t* emit_count,
-    uint32_t* counters, const int2* __restrict__ span, CollapseSummary* summary)
+    float4* c_bbmax, int4* st, uint32_t* emit_count, uint32_t* counters,
+    const int2* __restrict__ span, CollapseSummary* summary)
 {
     __shared__ uint32_t s_cnt[256];
     s_cnt[threadIdx.x] = 0u;
@@ -110,51 +119,33 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
     int node = (int)(N - 1 + i);
+    int cur = parent[node];
+    bool local = cur >= 0 && block_local(span[cur], cur, blk);
     {
         const float4 mn = bbmin[node];
         const float4 mx = bbmax[node];
-        c_parent[node] = parent[node];
+        c_parent[node] = cur;
         c_left[node] = left[node];
         c_right[node] = right[node];
         c_data[node] = data[node];
         c_bbmin[node] = mn;
         c_bbmax[node] = mx;
-        st_coh(&c_count[node], 1);
-        st_coh(&cost[node], ct * box_area4(mn, mx));
-        st_coh(&flags[node], 3u);
-        st_coh(&height[node], 0);
+        c_count[node] = 1;
+        put_state(st, node, make_int4(1, __float_as_int(ct * box_area4(mn, mx)), 3, 0),
+                  !local && cur >= 0);
     }
-    int cur = parent[node];
     while (cur >= 0) {
         // block-local nodes meet in LDS (workgroup scope), others globally
-        const bool local = block_local(span[cur], cur, blk);
         const uint32_t old =
             local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
         if (old == 0u) return;
         const int l = left[cur];
         const int r = right[cur];
-        int cl, cr, hl_, hr_;
-        float kl, kr;
-        uint32_t fl, fr;
-        if (local) {
-            cl = ld<false>(&c_count[l]);
-            cr = ld<false>(&c_count[r]);
-            kl = ld<false>(&cost[l]);
-            kr = ld<false>(&cost[r]);
-            fl = ld<false>(&flags[l]);
-            fr = ld<false>(&flags[r]);
-            hl_ = ld<false>(&height[l]);
-            hr_ = ld<false>(&height[r]);
-        } else {
-            cl = ld<true>(&c_count[l]);
-            cr = ld<true>(&c_count[r]);
-            kl = ld<true>(&cost[l]);
-            kr = ld<true>(&cost[r]);
-            fl = ld<true>(&flags[l]);
-            fr = ld<true>(&flags[r]);
-            hl_ = ld<true>(&height[l]);
-            hr_ = ld<true>(&height[r]);
-        }
+        const int4 a = local ? ld<false>(&st[l]) : ld<true>(&st[l]);
+        const int4 b = local ? ld<false>(&st[r]) : ld<true>(&st[r]);
+        const int cl = a.x, cr = b.x;
+        const float kl = __int_as_float(a.y), kr = __int_as_float(b.y);
+        const uint32_t fl = (uint32_t)a.z, fr = (uint32_t)b.z;
         const int count = cl + cr;
         const float4 mn = bbmin[cur];
         const float4 mx = bbmax[cur];
@@ -164,19 +155,17 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         const bool collapsible = asLeaf < cc;
         if (collapsible) cc = asLeaf;
         const bool both = (fl & 2u) && (fr & 2u);
-        st_coh(&flags[cur], (collapsible ? 1u : 0u) | ((both && collapsible) ? 2u : 0u));
+        const uint32_t f = (collapsible ? 1u : 0u) | ((both && collapsible) ? 2u : 0u);
         emit_count[cur] = (both && !collapsible)
                               ? (uint32_t)((cl > 1 ? cl : 0) + (cr > 1 ? cr : 0))
                               : 0u;
-        st_coh(&c_count[cur], count);
-        st_coh(&cost[cur], cc);
-        const int h = 1 + max(hl_, hr_);
-        st_coh(&height[cur], h);
+        const int h = 1 + max(a.w, b.w);
         const int pp = parent[cur];
         c_parent[cur] = pp;
         c_left[cur] = l;
         c_right[cur] = r;
         c_data[cur] = -1;
+        c_count[cur] = count;
         c_bbmin[cur] = mn;
         c_bbmax[cur] = mx;
         if (pp < 0) {
@@ -186,6 +175,8 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         }
         node = cur;
         cur = pp;
+        local = cur >= 0 && block_local(span[cur], cur, blk);
+        put_state(st, node, make_int4(count, __float_as_int(cc), (int)f, h), !local && cur >= 0);
     }
 }
 
@@ -333,15 +324,11 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
 {
     const uint32_t N = t->n;
     const size_t nodes = 2ull * N - 1;
-    float* cost = nullptr;
-    uint32_t* flags = nullptr;
-    int32_t* height = nullptr;
+    int4* state = nullptr;
     uint32_t* emit_count = nullptr;
     uint32_t* emit_off = nullptr;
     uint32_t* counters = nullptr;
-    ATR_TRY(dalloc_t(ctx, nodes, &cost));
-    ATR_TRY(dalloc_t(ctx, nodes, &flags));
-    ATR_TRY(dalloc_t(ctx, nodes, &height));
+    ATR_TRY(dalloc_t(ctx, nodes, &state));
     ATR_TRY(dalloc_t(ctx, N, &emit_count));
     ATR_TRY(dalloc_t(ctx, N, &emit_off));
     ATR_TRY(dalloc_t(ctx, N, &counters));
@@ -354,8 +341,8 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
     ATR_TRY(launch(ctx, "collapse_cost", (double)N * 2.0 * 100.0, [&] {
         k_collapse_cost<<<div_up(N, 256), 256, 0, ctx->stream>>>(
             N, ci, ct, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
-            c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, cost, flags,
-            height, emit_count, counters, t->span, c->d_summary);
+            c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, state,
+            emit_count, counters, t->span, c->d_summary);
     }));
     ATR_TRY(run_scan_u32(ctx, emit_count, N - 1, emit_off, &c->d_summary->used));
     if (verts) {
@@ -375,9 +362,7 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
                                                               c->bbmin, c->bbmax, c->tnodes);
         }));
     }
-    dfree(ctx, cost);
-    dfree(ctx, flags);
-    dfree(ctx, height);
+    dfree(ctx, state);
     dfree(ctx, emit_count);
     dfree(ctx, emit_off);
     dfree(ctx, counters);

@@ -138,6 +138,15 @@ __device__ __forceinline__ void st_coh(float4* p, float4 v)
     __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ void st_coh(int4* p, int4 v)
+{
+    uint64_t* q = (uint64_t*)p;
+    const uint64_t a = ((uint64_t)(uint32_t)v.y << 32) | (uint32_t)v.x;
+    const uint64_t b = ((uint64_t)(uint32_t)v.w << 32) | (uint32_t)v.z;
+    __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Loads: COH selects the device-coherent flavour (crossing readers).
 template <bool COH>
 __device__ __forceinline__ int32_t ld(const int32_t* p)
@@ -184,6 +193,20 @@ __device__ __forceinline__ float4 ld(const float4* p)
         const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return make_float4(__uint_as_float((uint32_t)a), __uint_as_float((uint32_t)(a >> 32)),
                            __uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+    } else {
+        return *p;
+    }
+}
+
+template <bool COH>
+__device__ __forceinline__ int4 ld(const int4* p)
+{
+    if constexpr (COH) {
+        const uint64_t* q = (const uint64_t*)p;
+        const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_int4((int)(uint32_t)a, (int)(uint32_t)(a >> 32), (int)(uint32_t)b,
+                         (int)(uint32_t)(b >> 32));
     } else {
         return *p;
     }

@@ -146,6 +146,7 @@ struct atr_batch {
     uint32_t* sort_status = nullptr;
     int32_t* tris = nullptr;
     float* nsah = nullptr;
+    int4* rec = nullptr;           // ATRBVH node records (n)
     uint32_t* counters = nullptr;
     float* d_sah = nullptr;        // per-mesh SAH
     bool built = false;
@@ -247,13 +248,18 @@ atr_status run_hit_adjoint(atr_context* ctx, const float4* v, uint32_t n_tris,
                            const float4* rays, const int4* hits, uint32_t R, const float* g_tuv,
                            const uint32_t* index, uint32_t n_grad, float* grad);
 atr_status run_depth_loss(atr_context* ctx, const int4* hits, const int4* target, uint32_t R,
-                          float* g_tuv, double* loss);
+                          int same_tri, float* g_tuv, double* loss);
 atr_status run_mesh_step(atr_context* ctx, float4* mesh, uint32_t V, float* grad, float lr,
-                         const uint32_t* index, uint32_t n_tris, float4* soup);
+                         float max_step, const uint32_t* index, uint32_t n_tris, float4* soup);
 
 // Treelets (treelet.hip)
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
-                           float ct, int32_t* tris, float* nsah, uint32_t* counters);
+                           float ct, int32_t* tris, float* nsah, uint32_t* counters,
+                           int4* rec);
+// Node records {left, right, area(left), area(right)} that the ATRBVH passes
+// read while forming treelets (first_leaf entries); built before the first
+// pass of every optimisation and kept current by the passes.
+atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec);
 atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
                           int32_t* tris, float* nsah, uint32_t* counters,
                           const int32_t* d_schedule, int rounds);

@@ -653,18 +653,21 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
     data[node] = (int32_t)tri;
     left[node] = -1;
     right[node] = -1;
-    // a sibling on another XCD may read this box at a crossing split
-    st_coh(&bbmin[node], mn);
-    st_coh(&bbmax[node], mx);
     area[node] = box_area4(mn, mx);
     span[node] = make_int2((int)i, (int)i);
 
+    // Every node's box is stored when its parent's split is known: with
+    // device-coherent stores when that split crosses workgroups (the sibling
+    // may read it from another XCD), with plain stores otherwise (the sibling
+    // takes it from LDS; later kernels see it after this one ends).
     int lo = (int)i, hi = (int)i;
     int dl = keys.delta(lo - 1);
     int dr = keys.delta(hi);
     while (true) {
         if (dl < 0 && dr < 0) {
             parent[node] = -1;
+            bbmin[node] = mn;
+            bbmax[node] = mx;
             break;
         }
         const bool is_left = dr > dl;
@@ -675,6 +678,8 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
         uint64_t old;
         float4 smn, smx;
         if (k >= 0 && k < 255 && s_local[k]) {
+            bbmin[node] = mn;
+            bbmax[node] = mx;
             float* my = s_box[k][is_left ? 0 : 1];
             my[0] = mn.x; my[1] = mn.y; my[2] = mn.z;
             my[3] = mx.x; my[4] = mx.y; my[5] = mx.z;
@@ -685,6 +690,8 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
             smn = make_float4(sb[0], sb[1], sb[2], 0.0f);
             smx = make_float4(sb[3], sb[4], sb[5], 0.0f);
         } else {
+            st_coh(&bbmin[node], mn);
+            st_coh(&bbmax[node], mx);
             old = exchange_crossing(&slots[split], mine);
             if (old == ~0ull) break;  // first arrival: the sibling finishes the parent
             const int sib = (int)(uint32_t)old;
@@ -710,8 +717,6 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
         data[p] = -1;
         parent[lchild] = p;
         parent[rchild] = p;
-        st_coh(&bbmin[p], mn);
-        st_coh(&bbmax[p], mx);
         area[p] = box_area4(mn, mx);
         span[p] = make_int2(lo, hi);
         node = p;
@@ -744,12 +749,20 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
                             fminf(fminf(a.z, b.z), c.z), 0.0f);
     float4 mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
                             fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
-    st_coh(&bbmin[node], mn);
-    st_coh(&bbmax[node], mx);
     area[node] = box_area4(mn, mx);
     int cur = parent[node];
+    // A box is read during this launch only by the sibling's thread at the
+    // parent; it needs a device-coherent store only when that hand-over
+    // crosses workgroups.
+    bool local = cur >= 0 && block_local(span[cur], cur, blk);
+    if (local || cur < 0) {
+        bbmin[node] = mn;
+        bbmax[node] = mx;
+    } else {
+        st_coh(&bbmin[node], mn);
+        st_coh(&bbmax[node], mx);
+    }
     while (cur >= 0) {
-        const bool local = block_local(span[cur], cur, blk);
         const uint32_t old =
             local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
         if (old == 0u) return;
@@ -758,11 +771,17 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
         const float4 smx = local ? ld<false>(&bbmax[other]) : ld<true>(&bbmax[other]);
         mn = make_float4(fminf(mn.x, smn.x), fminf(mn.y, smn.y), fminf(mn.z, smn.z), 0.0f);
         mx = make_float4(fmaxf(mx.x, smx.x), fmaxf(mx.y, smx.y), fmaxf(mx.z, smx.z), 0.0f);
-        st_coh(&bbmin[cur], mn);
-        st_coh(&bbmax[cur], mx);
         area[cur] = box_area4(mn, mx);
         node = cur;
         cur = parent[cur];
+        local = cur >= 0 && block_local(span[cur], cur, blk);
+        if (local || cur < 0) {
+            bbmin[node] = mn;
+            bbmax[node] = mx;
+        } else {
+            st_coh(&bbmin[node], mn);
+            st_coh(&bbmax[node], mx);
+        }
     }
 }
 

@@ -94,6 +94,55 @@ __device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int r
     }
 }
 
+// formTreelet as above, reading each expanded node's children and their
+// areas with one 16-byte load of its record {left, right, area(left),
+// area(right)} (kept by the ATRBVH pass next to the SoA arrays) instead of
+// two dependent loads.  The choices are identical: the record holds the
+// values the arrays hold.
+template <bool COH>
+__device__ __forceinline__ void form_treelet_rec(const HalfTreelet& h, int hl, int root, int n,
+                                                 int first_leaf, const int4* rec)
+{
+    if (hl == 0) {
+        const int4 q = ld<COH>(&rec[root]);
+        h.internal[0] = root;
+        h.leaf[0] = q.x;
+        h.leaf[1] = q.y;
+        h.area[0] = __int_as_float(q.z);
+        h.area[1] = __int_as_float(q.w);
+    }
+    for (int it = 0; it < n - 2; ++it) {
+        const int m = it + 2;
+        int idx = hl;
+        float a = -INFINITY;
+        if (hl < m) {
+            a = h.area[hl];
+            if (h.leaf[hl] >= first_leaf) a = -FLT_MAX;
+        }
+        int cnt = m;
+        int sh = m >> 1;
+        while (cnt > 1) {
+            const int oi = __shfl_down(idx, sh, 32);
+            const float oa = __shfl_down(a, sh, 32);
+            if (oa > a) {
+                a = oa;
+                idx = oi;
+            }
+            cnt = (cnt + 1) >> 1;
+            sh = cnt >> 1;
+        }
+        if (hl == 0) {
+            const int rep = h.leaf[idx];
+            const int4 q = ld<COH>(&rec[rep]);
+            h.internal[it + 1] = rep;
+            h.leaf[idx] = q.x;
+            h.leaf[it + 2] = q.y;
+            h.area[idx] = __int_as_float(q.z);
+            h.area[it + 2] = __int_as_float(q.w);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ ATRBVH
 // Per-half LDS words: internal 32 + leaf 32 + area 32 + boxes 32*6 + spans
 // 32*2 + matrix.
@@ -104,18 +153,23 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
                                                int first_leaf, float ci, int32_t* parent,
                                                int32_t* left, int32_t* right, float4* bbmin,
                                                float4* bbmax, float* area, float* nsah,
-                                               int2* span)
+                                               int2* span, int4* rec)
 {
     HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 32, lds + 64};
     float* box = lds + 96;
     int2* sp = (int2*)(lds + 288);
     float* D = lds + kAggFixedWords;
-    form_treelet<COH>(h, hl, root, n, first_leaf, left, right, area);
+    form_treelet_rec<COH>(h, hl, root, n, first_leaf, rec);
 
+    // Each lane also carries the area of its current node (`my_area`), so
+    // that every rewritten node's record gets its children's areas.
     int node = -1;
     float sah = 0.0f;
+    float my_area = 0.0f;
+    float rec_al = 0.0f, rec_ar = 0.0f;
     if (hl < n) {
         node = h.leaf[hl];
+        my_area = h.area[hl];
         sah = ld<COH>(&nsah[node]);
         const float4 a = ld<COH>(&bbmin[node]);
         const float4 b = ld<COH>(&bbmax[node]);
@@ -167,6 +221,8 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
         const float s = __shfl(sah, r, 32) + __shfl(sah, c, 32);
         const int li = __shfl(node, r, 32);
         const int ri = __shfl(node, c, 32);
+        const float ali = __shfl(my_area, r, 32);
+        const float ari = __shfl(my_area, c, 32);
         float amn[3], amx[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -187,9 +243,11 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
             const float v = s + ci * ar;
             h.area[L] = v;
             sah = v;
+            my_area = ar;
         }
         const int lastIndex = __shfl(node, L, 32);
         const float sahLast = __shfl(sah, L, 32);
+        const float areaLast = __shfl(my_area, L, 32);
         float bmn[3], bmx[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -200,6 +258,7 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
         if (hl == r) {
             node = lastIndex;
             sah = sahLast;
+            my_area = areaLast;
             sp[r] = spL;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -210,6 +269,8 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
         if (hl == L) {
             node = li;
             h.leaf[L] = ri;
+            rec_al = ali;
+            rec_ar = ari;
             sp[L] = spm;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -249,6 +310,7 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
             st_coh(&bbmax[ni], mx);
             st_coh(&area[ni], box_area4(mn, mx));
             st_coh(&span[ni], sp[hl]);
+            st_coh(&rec[ni], make_int4(node, rr, __float_as_int(rec_al), __float_as_int(rec_ar)));
         }
     }
 }
@@ -493,8 +555,9 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
                                                       int32_t* right, float4* bbmin,
                                                       float4* bbmax, float* area, int32_t* tris,
                                                       float* nsah, uint32_t* counters,
-                                                      int2* span, const int32_t* sched,
-                                                      int rounds, int half_words)
+                                                      int2* span, int4* rec,
+                                                      const int32_t* sched, int rounds,
+                                                      int half_words)
 {
     extern __shared__ float smem[];
     __shared__ uint32_t s_cnt[256];
@@ -571,10 +634,10 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
                 if (AGG) {
                     if (root_local)
                         atrbvh_treelet<false>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
-                                              right, bbmin, bbmax, area, nsah, span);
+                                              right, bbmin, bbmax, area, nsah, span, rec);
                     else
                         atrbvh_treelet<true>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
-                                             right, bbmin, bbmax, area, nsah, span);
+                                             right, bbmin, bbmax, area, nsah, span, rec);
                 } else {
                     if (root_local)
                         trbvh_treelet<false>(lds, hl, root, root_sah, root_tris, first_leaf, ci, ct, sched,
@@ -607,10 +670,39 @@ atr_status zero_counters(atr_context* ctx, uint32_t* counters, uint32_t count)
     });
 }
 
+// Node records {left, right, area(left), area(right)} of every internal node
+// from the SoA arrays.  Internal slots whose links are not node indices (the
+// spare slot of each mesh in a batch's forest) get {-1, -1, 0, 0}; no treelet
+// reaches them.
+__global__ __launch_bounds__(256) void k_make_records(uint32_t first_leaf, uint32_t nodes,
+                                                      const int32_t* __restrict__ left,
+                                                      const int32_t* __restrict__ right,
+                                                      const float* __restrict__ area,
+                                                      int4* __restrict__ rec)
+{
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= first_leaf) return;
+    const int l = left[x];
+    const int r = right[x];
+    int4 q = make_int4(-1, -1, 0, 0);
+    if ((uint32_t)l < nodes && (uint32_t)r < nodes)
+        q = make_int4(l, r, __float_as_int(area[l]), __float_as_int(area[r]));
+    rec[x] = q;
+}
+
 }  // namespace
 
+atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec)
+{
+    const uint32_t fl = t->first_leaf;
+    return launch(ctx, "make_records", (double)fl * 32.0, [&] {
+        k_make_records<<<std::max(div_up(fl, 256), 1u), 256, 0, ctx->stream>>>(
+            fl, fl + t->n, t->left, t->right, t->area, rec);
+    });
+}
+
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
-                           float ct, int32_t* tris, float* nsah, uint32_t* counters)
+                           float ct, int32_t* tris, float* nsah, uint32_t* counters, int4* rec)
 {
     const uint32_t N = t->n;
     const int first_leaf = (int)t->first_leaf;
@@ -620,7 +712,7 @@ atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma
     return launch(ctx, "atrbvh_pass", 0.0, [&] {
         k_treelet_pass<true><<<div_up(N, 256), 256, lds, ctx->stream>>>(
             N, first_leaf, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin,
-            t->bbmax, t->area, tris, nsah, counters, t->span, nullptr, 0, half_words);
+            t->bbmax, t->area, tris, nsah, counters, t->span, rec, nullptr, 0, half_words);
     });
 }
 
@@ -635,7 +727,7 @@ atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, fl
     return launch(ctx, "trbvh_pass", 0.0, [&] {
         k_treelet_pass<false><<<div_up(N, 256), 256, lds, ctx->stream>>>(
             N, first_leaf, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax,
-            t->area, tris, nsah, counters, t->span, d_schedule, rounds, kTrWords);
+            t->area, tris, nsah, counters, t->span, nullptr, d_schedule, rounds, kTrWords);
     });
 }
 

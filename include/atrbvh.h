@@ -230,16 +230,21 @@ atr_status atr_hit_adjoint(atr_context* ctx, atr_scene* scene, const float* d_ra
                            const int32_t* d_hits, uint32_t num_rays, const float* d_grad_tuv,
                            const uint32_t* d_index, uint32_t num_grad_vertices, float* d_grad);
 /* Depth loss of the differentiable-refit workload: adds sum (t - t_target)^2
- * over rays hit in both d_hits and d_target_hits to *d_loss (device double)
- * and writes d_grad_tuv = (2 (t - t_target), 0, 0) per ray (zero elsewhere). */
+ * over rays hit in both d_hits and d_target_hits (and, when same_triangle is
+ * nonzero, on the same triangle in both) to *d_loss (device double) and
+ * writes d_grad_tuv = (2 (t - t_target), 0, 0) per such ray (zero elsewhere). */
 atr_status atr_depth_loss(atr_context* ctx, const int32_t* d_hits, const int32_t* d_target_hits,
-                          uint32_t num_rays, float* d_grad_tuv, double* d_loss);
-/* Vertex step on an indexed mesh: d_mesh_vertices (float4[num_vertices])
- * -= lr * d_grad (float[3 num_vertices], then cleared; d_grad NULL => no
- * step), then the scene's triangle soup is gathered from the mesh through
- * d_index (uint32[3n]).  Follow with atr_refit to update the tree. */
+                          uint32_t num_rays, int same_triangle, float* d_grad_tuv,
+                          double* d_loss);
+/* Vertex step on an indexed mesh: every vertex of d_mesh_vertices
+ * (float4[num_vertices]) moves by -lr * d_grad (float[3 num_vertices]),
+ * shortened to length max_step when it is longer (max_step <= 0: no limit);
+ * d_grad is then cleared (d_grad NULL => no step).  The scene's triangle soup
+ * is then gathered from the mesh through d_index (uint32[3n]).  Follow with
+ * atr_refit to update the tree. */
 atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_vertices,
-                         float* d_grad, float lr, const uint32_t* d_index, atr_scene* scene);
+                         float* d_grad, float lr, float max_step, const uint32_t* d_index,
+                         atr_scene* scene);
 
 /* SAH of a tree into device memory, asynchronously, accumulated in double
  * precision by a parallel reduction (for monitoring such as a rebuild
@@ -299,9 +304,10 @@ atr_status atr_batch_trace(atr_batch* b, const float* d_rays, uint32_t rays_per_
  * slot per triangle corner, accumulated). */
 atr_status atr_batch_hit_adjoint(atr_batch* b, const float* d_rays, const int32_t* d_hits,
                                  uint32_t num_rays, const float* d_grad_tuv, float* d_grad);
-/* Vertex step on the batch's soup: vertices -= lr * d_grad (float[9T], then
- * cleared). */
-atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr);
+/* Vertex step on the batch's soup: every corner moves by -lr * d_grad
+ * (float[9T]), shortened to length max_step when longer (max_step <= 0: no
+ * limit); d_grad is then cleared. */
+atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr, float max_step);
 /* Bottom-up refit of every mesh's tree over the batch's current vertices. */
 atr_status atr_batch_refit(atr_batch* b);
 /* Rebuilds (LBVH 30-bit + ATRBVH) only the listed meshes from their current

@@ -107,6 +107,54 @@ def test_generic_tuv_adjoint(ctx, sphere):
     assert rel_err(acc, ref) < TOL
 
 
+def test_capped_vertex_step_and_same_triangle_loss(ctx, sphere):
+    """A capped step moves no vertex further than max_step and is the plain
+    fp32 step wherever that is shorter (bit for bit, same operation order);
+    the same-triangle depth loss counts only rays whose hit triangle equals
+    the target's, with upstream gradient 2 (t - t_target) there and 0
+    elsewhere."""
+    from helpers import bits
+
+    m, R = sphere["m"], sphere["R"]
+    V = m["vertices"].shape[0]
+    g_gpu, _ = _gpu_loss_grad(ctx, sphere)
+    s = ctx.scene(wl.soup(m["vertices"], m["index"]))
+    d_mesh = ctx.to_device(wl.mesh_float4(m["vertices"]))
+    d_grad = ctx.to_device(g_gpu)
+    d_index = ctx.to_device(m["index"].astype(np.uint32).reshape(-1))
+    lr, cap = np.float32(0.01), np.float32(0.002)
+    ctx.mesh_step(d_mesh, V, d_grad, float(lr), d_index, s, float(cap))
+    mesh = d_mesh.download(np.float32, (V, 4))
+    step = lr * g_gpu
+    length = np.sqrt(step[:, 0] * step[:, 0] + step[:, 1] * step[:, 1] + step[:, 2] * step[:, 2])
+    long_ = length > cap
+    assert long_.any() and (~long_).any()
+    scale = np.ones(V, np.float32)
+    scale[long_] = cap / length[long_]
+    step[long_] *= scale[long_, None]
+    want = wl.mesh_float4(m["vertices"])
+    want[:, :3] -= step
+    assert np.array_equal(bits(mesh), bits(want))
+    assert np.linalg.norm(step.astype(np.float64), axis=1).max() <= float(cap) * (1 + 1e-5)
+
+    hits = sphere["d_hits"].download(np.int32, (R, 4))
+    target = sphere["d_target"].download(np.int32, (R, 4))
+    d_g = ctx.buffer(R * 12)
+    d_loss = ctx.to_device(np.zeros(1, np.float64))
+    ctx.depth_loss(sphere["d_hits"], sphere["d_target"], R, d_g, d_loss, same_triangle=True)
+    g = d_g.download(np.float32, (R, 3))
+    loss = float(d_loss.download(np.float64, 1)[0])
+    same = (hits[:, 0] >= 0) & (target[:, 0] >= 0) & (hits[:, 0] == target[:, 0])
+    diff_t = hits[:, 1].view(np.float32) - target[:, 1].view(np.float32)
+    assert same.any() and (~same & (hits[:, 0] >= 0) & (target[:, 0] >= 0)).any()
+    want_g = np.where(same, np.float32(2.0) * diff_t, np.float32(0.0)).astype(np.float32)
+    assert np.array_equal(bits(g[:, 0]), bits(want_g))
+    assert not np.any(g[:, 1:])
+    want_loss = float(np.sum(diff_t[same].astype(np.float64) ** 2))
+    assert abs(loss - want_loss) <= 1e-12 * max(want_loss, 1.0)
+    s.close()
+
+
 def test_refit_after_vertex_step_matches_rebuild_boxes(ctx, sphere):
     """One gradient step on the indexed mesh, gathered into the scene's soup,
     then refit: every box equals a recursive union over the moved vertices."""
