@@ -20,21 +20,62 @@ namespace atr {
 namespace {
 
 // (row << 8 | col) of entry p of a lower-triangular matrix stored row by row
-// without its diagonal: p = row*(row-1)/2 + col (TriangularMatrix.cuh:34-52).
-struct TrmTable {
-    uint16_t rc[496];
+// without its diagonal: p = row*(row-1)/2 + col (TriangularMatrix.cuh:34-52),
+// in 32-bit words so that wave-uniform lookups are scalar loads.
+struct TrmTable32 {
+    uint32_t rc[512];
 };
-constexpr TrmTable make_trm()
+constexpr TrmTable32 make_trm32()
 {
-    TrmTable t{};
+    TrmTable32 t{};
     int p = 0;
     for (int r = 1; r < 32; ++r)
-        for (int c = 0; c < r; ++c) t.rc[p++] = (uint16_t)((r << 8) | c);
+        for (int c = 0; c < r; ++c) t.rc[p++] = (uint32_t)((r << 8) | c);
     return t;
 }
-__constant__ TrmTable c_trm = make_trm();
+__constant__ TrmTable32 c_trm32 = make_trm32();
 
 __device__ __forceinline__ int trm(int row, int col) { return ((row - 1) * row) / 2 + col; }
+
+// Row of entry p of the same matrix in registers: the hardware square root is
+// within one row of the answer for every p < 496 and the integer checks make
+// it exact.
+__device__ __forceinline__ int trm_row(int p)
+{
+    int r = (int)((1.0f + __builtin_amdgcn_sqrtf((float)(8 * p + 1))) * 0.5f);
+    r -= (r * (r - 1)) / 2 > p ? 1 : 0;
+    r += (r * (r + 1)) / 2 <= p ? 1 : 0;
+    return r;
+}
+
+// Cross-lane helpers for two 32-lane halves working on different treelets.
+// bcast_half: the value of lane k0 (lower half) or k1 (upper half) of the
+// caller's own half; k0 and k1 are wave-uniform.
+__device__ __forceinline__ bool upper_half() { return (__lane_id() & 32u) != 0u; }
+__device__ __forceinline__ int bcast_half(int v, int k0, int k1, bool upper)
+{
+    const int a = __builtin_amdgcn_readlane(v, k0 & 31);
+    const int b = __builtin_amdgcn_readlane(v, 32 + (k1 & 31));
+    return upper ? b : a;
+}
+__device__ __forceinline__ float bcast_half(float v, int k0, int k1, bool upper)
+{
+    return __int_as_float(bcast_half(__float_as_int(v), k0, k1, upper));
+}
+// One step of a __shfl_down halving tree at distance S < 16: lane i takes
+// lane i + S of its 16-lane row (lanes whose partner is outside the row keep
+// their own value, which the strict comparison then leaves unchanged).
+template <int S>
+__device__ __forceinline__ void min_step_row(float& md, int& mi)
+{
+    const float om = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(md), __float_as_int(md), 0x100 | S, 0xF, 0xF, false));
+    const int oi = __builtin_amdgcn_update_dpp(mi, mi, 0x100 | S, 0xF, 0xF, false);
+    if (om < md) {
+        md = om;
+        mi = oi;
+    }
+}
 
 // Per-half LDS layout shared by both kernels' treelet formation.
 struct HalfTreelet {
@@ -184,13 +225,14 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
     // calculateDistancesMatrix (AgglomerativeTreelet.cu:18-63): every pair i > j
     const int tsize = n * (n - 1) / 2;
     for (int p = hl; p < tsize; p += 32) {
-        const int rc = c_trm.rc[p];
-        const float* bi = box + (rc >> 8) * 6;
-        const float* bj = box + (rc & 255) * 6;
+        const int pr = trm_row(p);
+        const float* bi = box + pr * 6;
+        const float* bj = box + (p - (pr * (pr - 1)) / 2) * 6;
         D[p] = union_area(bi[0], bi[1], bi[2], bi[3], bi[4], bi[5], bj[0], bj[1], bj[2], bj[3],
                           bj[4], bj[5]);
     }
 
+    const bool upper = upper_half();
     for (int L = n - 1; L > 0; --L) {
         // findMinimumDistance (:83-99)
         const int size = L * (L + 1) / 2;
@@ -203,26 +245,40 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
                 mi = j;
             }
         }
-#pragma unroll
-        for (int sh = 16; sh > 0; sh >>= 1) {
-            const float om = __shfl_down(md, sh, 32);
-            const int oi = __shfl_down(mi, sh, 32);
+        // The width-32 __shfl_down halving tree (distances 16, 8, 4, 2, 1):
+        // only lanes i < distance feed lane 0's result, and each of them takes
+        // lane i + distance, so distance 16 is an xor-16 swizzle inside each
+        // 32-lane group and the rest are DPP row shifts.
+        {
+            const float om =
+                __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(md), 0x401F));
+            const int oi = __builtin_amdgcn_ds_swizzle(mi, 0x401F);
             if (om < md) {
                 md = om;
                 mi = oi;
             }
         }
-        const int minIndex = __shfl(mi, 0, 32);
-        const int rc = c_trm.rc[minIndex];
-        const int r = rc >> 8;
-        const int c = rc & 255;
+        min_step_row<8>(md, mi);
+        min_step_row<4>(md, mi);
+        min_step_row<2>(md, mi);
+        min_step_row<1>(md, mi);
+        // Each half's minimum index, as wave-uniform values (an idle upper
+        // half yields a stale index, clamped into the table and never used).
+        const int mi0 = __builtin_amdgcn_readlane(mi, 0);
+        const int mi1 = __builtin_amdgcn_readlane(mi, 32);
+        const int rc0 = (int)c_trm32.rc[(uint32_t)mi0 & 511u];
+        const int rc1 = (int)c_trm32.rc[(uint32_t)mi1 & 511u];
+        const int r0 = rc0 >> 8, c0 = rc0 & 255;
+        const int r1 = rc1 >> 8, c1 = rc1 & 255;
+        const int r = upper ? r1 : r0;
+        const int c = upper ? c1 : c0;
 
         // updateState (:101-150)
-        const float s = __shfl(sah, r, 32) + __shfl(sah, c, 32);
-        const int li = __shfl(node, r, 32);
-        const int ri = __shfl(node, c, 32);
-        const float ali = __shfl(my_area, r, 32);
-        const float ari = __shfl(my_area, c, 32);
+        const float s = bcast_half(sah, r0, r1, upper) + bcast_half(sah, c0, c1, upper);
+        const int li = bcast_half(node, r0, r1, upper);
+        const int ri = bcast_half(node, c0, c1, upper);
+        const float ali = bcast_half(my_area, r0, r1, upper);
+        const float ari = bcast_half(my_area, c0, c1, upper);
         float amn[3], amx[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -245,9 +301,9 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
             sah = v;
             my_area = ar;
         }
-        const int lastIndex = __shfl(node, L, 32);
-        const float sahLast = __shfl(sah, L, 32);
-        const float areaLast = __shfl(my_area, L, 32);
+        const int lastIndex = bcast_half(node, L, L, upper);
+        const float sahLast = bcast_half(sah, L, L, upper);
+        const float areaLast = bcast_half(my_area, L, L, upper);
         float bmn[3], bmx[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
