@@ -43,10 +43,17 @@ SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "batch.hip", 
            "api.cpp",
            "scenes.cpp", "schedule.cpp"]
 
+# Per-file flags.  treelet.hip only ever handles finite boxes, areas and SAH
+# values (scenes reject non-finite vertices), so fminf/fmaxf there need no
+# NaN quieting of their operands: without it the agglomeration loop issues
+# ~12% fewer VALU instructions, with identical results.  The traversal keeps
+# IEEE NaN handling (its slab tests can form 0 * inf).
+EXTRA = {"treelet.hip": ["-fno-honor-nans"]}
+
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, *DEVICE]
+    cmd = [HIPCC, *COMMON, *DEVICE, *EXTRA.get(src, [])]
     if src.endswith(".hip"):
         cmd += ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -108,7 +108,8 @@ atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* laun
  * at FLT_MIN (gpu-ray-traversal/src/rt/SceneConverter.cpp:28-64). */
 atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t num_triangles,
                             const float* bbmin, const float* bbmax, atr_scene** out);
-/* Same, from vertices already resident on the device (copied device-to-device). */
+/* Same, from vertices already resident on the device (copied device-to-device).
+ * Coordinates must be finite: host arrays are checked, device arrays are not. */
 atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices,
                                    uint32_t num_triangles, const float* bbmin,
                                    const float* bbmax, atr_scene** out);
