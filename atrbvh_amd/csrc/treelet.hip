@@ -62,19 +62,28 @@ __device__ __forceinline__ float bcast_half(float v, int k0, int k1, bool upper)
 {
     return __int_as_float(bcast_half(__float_as_int(v), k0, k1, upper));
 }
-// One step of a __shfl_down halving tree at distance S < 16: lane i takes
-// lane i + S of its 16-lane row (lanes whose partner is outside the row keep
-// their own value, which the strict comparison then leaves unchanged).
+// Minimum of v over lanes i, i + S of each 16-lane row (DPP row shift).  A
+// lane whose partner lies outside the row gets an unspecified value; such
+// lanes never feed lane 0 of the row, the only lane read afterwards.
 template <int S>
-__device__ __forceinline__ void min_step_row(float& md, int& mi)
+__device__ __forceinline__ float min_row(float v)
 {
-    const float om = __int_as_float(
-        __builtin_amdgcn_update_dpp(__float_as_int(md), __float_as_int(md), 0x100 | S, 0xF, 0xF, false));
-    const int oi = __builtin_amdgcn_update_dpp(mi, mi, 0x100 | S, 0xF, 0xF, false);
-    if (om < md) {
-        md = om;
-        mi = oi;
+    return fminf(v, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x100 | S, 0xF, 0xF,
+                                                             false)));
+}
+
+// Among the lanes of a 32-bit lane mask, the one with the smallest 5-bit
+// reversed index: lanes with bit 0 clear first, then bit 1, ... (0 if the
+// mask is empty).
+__device__ __forceinline__ int lane_min_br5(uint32_t m)
+{
+    const uint32_t pat[5] = {0x55555555u, 0x33333333u, 0x0F0F0F0Fu, 0x00FF00FFu, 0x0000FFFFu};
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const uint32_t t = m & pat[b];
+        m = t ? t : m;
     }
+    return m ? __builtin_ctz(m) : 0;
 }
 
 // Per-half LDS layout shared by both kernels' treelet formation.
@@ -245,27 +254,26 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
                 mi = j;
             }
         }
-        // The width-32 __shfl_down halving tree (distances 16, 8, 4, 2, 1):
-        // only lanes i < distance feed lane 0's result, and each of them takes
-        // lane i + distance, so distance 16 is an xor-16 swizzle inside each
-        // 32-lane group and the rest are DPP row shifts.
-        {
-            const float om =
-                __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(md), 0x401F));
-            const int oi = __builtin_amdgcn_ds_swizzle(mi, 0x401F);
-            if (om < md) {
-                md = om;
-                mi = oi;
-            }
-        }
-        min_step_row<8>(md, mi);
-        min_step_row<4>(md, mi);
-        min_step_row<2>(md, mi);
-        min_step_row<1>(md, mi);
+        // The reference's width-32 __shfl_down halving tree with strict
+        // comparisons picks the smallest lane value and, among equal values,
+        // the lane with the smallest 5-bit-reversed index (the last step
+        // prefers even lanes, the one before lanes 0 mod 4, ...).  Here: the
+        // minimum value of each half (order-free: xor-16 swizzle, then DPP
+        // row shifts into lane 0), a ballot of the lanes holding it, and the
+        // winning lane picked from the ballot on the scalar unit.
+        float mv = fminf(md, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(md), 0x401F)));
+        mv = min_row<8>(mv);
+        mv = min_row<4>(mv);
+        mv = min_row<2>(mv);
+        mv = min_row<1>(mv);
+        const float m0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mv), 0));
+        const float m1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mv), 32));
+        const uint32_t w0 = (uint32_t)__ballot(md == m0);
+        const uint32_t w1 = (uint32_t)(__ballot(md == m1) >> 32);
         // Each half's minimum index, as wave-uniform values (an idle upper
         // half yields a stale index, clamped into the table and never used).
-        const int mi0 = __builtin_amdgcn_readlane(mi, 0);
-        const int mi1 = __builtin_amdgcn_readlane(mi, 32);
+        const int mi0 = __builtin_amdgcn_readlane(mi, lane_min_br5(w0));
+        const int mi1 = __builtin_amdgcn_readlane(mi, 32 + lane_min_br5(w1));
         const int rc0 = (int)c_trm32.rc[(uint32_t)mi0 & 511u];
         const int rc1 = (int)c_trm32.rc[(uint32_t)mi1 & 511u];
         const int r0 = rc0 >> 8, c0 = rc0 & 255;

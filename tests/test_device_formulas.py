@@ -47,3 +47,40 @@ def test_trm_row_matches_row_major_enumeration():
             assert p - r * (r - 1) // 2 == c
             p += 1
     assert p == 496
+
+
+def _br5(lane: int) -> int:
+    return int(f"{lane:05b}"[::-1], 2)
+
+
+def _min_by_ballot(D, last_row: int) -> int:
+    """The selection rule the ATRBVH kernel uses for findMinimum: every lane
+    l of a 32-lane half keeps its first strict minimum over entries
+    l, l + 32, ...; the winner is the smallest lane value, ties going to the
+    lane with the smallest 5-bit-reversed index."""
+    size = last_row * (last_row + 1) // 2
+    best = []
+    for lane in range(32):
+        md, mi = np.float32(np.finfo(np.float32).max), 0
+        for j in range(lane, size, 32):
+            if D[j] < md:
+                md, mi = D[j], j
+        best.append((md, mi))
+    m = min(b[0] for b in best)
+    lane = min((l for l in range(32) if best[l][0] == m), key=_br5)
+    return best[lane][1]
+
+
+def test_min_by_ballot_equals_halving_tree_on_ties():
+    """The oracle emulates the reference's width-32 __shfl_down halving tree
+    (AgglomerativeTreelet.cu:65-99); the kernel's ballot formulation must pick
+    the same index, ties included (few distinct values force many ties)."""
+    import oracle
+
+    rng = np.random.default_rng(11)
+    for trial in range(3000):
+        last_row = int(rng.integers(1, 32))
+        size = last_row * (last_row + 1) // 2
+        levels = int(rng.integers(1, 5))
+        D = rng.integers(0, levels, size).astype(np.float32) * np.float32(0.25)
+        assert _min_by_ballot(D, last_row) == oracle.find_minimum(D, last_row), trial
