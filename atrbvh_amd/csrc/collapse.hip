@@ -97,6 +97,10 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // parent needs from a child -- triangle count, cost, flags, height -- is one
 // 16-byte record, stored device-coherently only when the parent's hand-over
 // crosses workgroups (the other child's thread may then run on another XCD).
+// The copy of the tree's links and boxes does not depend on the climb: thread
+// i copies internal node i and leaf i up front, coalesced; the climb writes
+// only the counts.  A node's area is computed from its box, as the reference
+// does (TreeCollapser.cu:50-51), not taken from the tree's area array.
 __device__ __forceinline__ void put_state(int4* st, int node, int4 v, bool coherent)
 {
     if (coherent) st_coh(&st[node], v);
@@ -118,6 +122,14 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     const int blk = (int)blockIdx.x;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
+    if (i + 1 < N) {  // internal node i
+        c_parent[i] = parent[i];
+        c_left[i] = left[i];
+        c_right[i] = right[i];
+        c_data[i] = -1;
+        c_bbmin[i] = bbmin[i];
+        c_bbmax[i] = bbmax[i];
+    }
     int node = (int)(N - 1 + i);
     int cur = parent[node];
     bool local = cur >= 0 && block_local(span[cur], cur, blk);
@@ -147,9 +159,7 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         const float kl = __int_as_float(a.y), kr = __int_as_float(b.y);
         const uint32_t fl = (uint32_t)a.z, fr = (uint32_t)b.z;
         const int count = cl + cr;
-        const float4 mn = bbmin[cur];
-        const float4 mx = bbmax[cur];
-        const float ar = box_area4(mn, mx);
+        const float ar = box_area4(bbmin[cur], bbmax[cur]);
         float cc = ci * ar + kl + kr;
         const float asLeaf = ct * ar * (float)count;
         const bool collapsible = asLeaf < cc;
@@ -161,13 +171,7 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
                               : 0u;
         const int h = 1 + max(a.w, b.w);
         const int pp = parent[cur];
-        c_parent[cur] = pp;
-        c_left[cur] = l;
-        c_right[cur] = r;
-        c_data[cur] = -1;
         c_count[cur] = count;
-        c_bbmin[cur] = mn;
-        c_bbmax[cur] = mx;
         if (pp < 0) {
             summary->cost_root = cc;
             summary->area_root = tarea[cur];
