@@ -132,7 +132,21 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     }
     int node = (int)(N - 1 + i);
     int cur = parent[node];
-    bool local = cur >= 0 && block_local(span[cur], cur, blk);
+    // The next node's span, links, parent and box are loaded together as soon
+    // as it is known (they do not change during the pass).
+    int2 nsp = make_int2(0, 0);
+    int nl = -1, nr = -1, npp = -1;
+    float4 nmn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nmx = nmn;
+    auto fetch = [&](int x) {
+        nsp = span[x];
+        nl = left[x];
+        nr = right[x];
+        npp = parent[x];
+        nmn = bbmin[x];
+        nmx = bbmax[x];
+    };
+    if (cur >= 0) fetch(cur);
+    bool local = cur >= 0 && block_local(nsp, cur, blk);
     {
         const float4 mn = bbmin[node];
         const float4 mx = bbmax[node];
@@ -151,15 +165,15 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         const uint32_t old =
             local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
         if (old == 0u) return;
-        const int l = left[cur];
-        const int r = right[cur];
+        const int l = nl;
+        const int r = nr;
         const int4 a = local ? ld<false>(&st[l]) : ld<true>(&st[l]);
         const int4 b = local ? ld<false>(&st[r]) : ld<true>(&st[r]);
         const int cl = a.x, cr = b.x;
         const float kl = __int_as_float(a.y), kr = __int_as_float(b.y);
         const uint32_t fl = (uint32_t)a.z, fr = (uint32_t)b.z;
         const int count = cl + cr;
-        const float ar = box_area4(bbmin[cur], bbmax[cur]);
+        const float ar = box_area4(nmn, nmx);
         float cc = ci * ar + kl + kr;
         const float asLeaf = ct * ar * (float)count;
         const bool collapsible = asLeaf < cc;
@@ -170,7 +184,7 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
                               ? (uint32_t)((cl > 1 ? cl : 0) + (cr > 1 ? cr : 0))
                               : 0u;
         const int h = 1 + max(a.w, b.w);
-        const int pp = parent[cur];
+        const int pp = npp;
         c_count[cur] = count;
         if (pp < 0) {
             summary->cost_root = cc;
@@ -179,7 +193,8 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         }
         node = cur;
         cur = pp;
-        local = cur >= 0 && block_local(span[cur], cur, blk);
+        if (cur >= 0) fetch(cur);
+        local = cur >= 0 && block_local(nsp, cur, blk);
         put_state(st, node, make_int4(count, __float_as_int(cc), (int)f, h), !local && cur >= 0);
     }
 }

@@ -751,10 +751,20 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
                             fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
     area[node] = box_area4(mn, mx);
     int cur = parent[node];
-    // A box is read during this launch only by the sibling's thread at the
-    // parent; it needs a device-coherent store only when that hand-over
-    // crosses workgroups.
-    bool local = cur >= 0 && block_local(span[cur], cur, blk);
+    // The next node's span, links and parent are loaded together as soon as
+    // it is known (the topology is fixed).  A box is read during this launch
+    // only by the sibling's thread at the parent; it needs a device-coherent
+    // store only when that hand-over crosses workgroups.
+    int2 nsp = make_int2(0, 0);
+    int nl = -1, nr = -1, npp = -1;
+    auto fetch = [&](int x) {
+        nsp = span[x];
+        nl = left[x];
+        nr = right[x];
+        npp = parent[x];
+    };
+    if (cur >= 0) fetch(cur);
+    bool local = cur >= 0 && block_local(nsp, cur, blk);
     if (local || cur < 0) {
         bbmin[node] = mn;
         bbmax[node] = mx;
@@ -766,15 +776,16 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
         const uint32_t old =
             local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
         if (old == 0u) return;
-        const int other = left[cur] == node ? right[cur] : left[cur];
+        const int other = nl == node ? nr : nl;
         const float4 smn = local ? ld<false>(&bbmin[other]) : ld<true>(&bbmin[other]);
         const float4 smx = local ? ld<false>(&bbmax[other]) : ld<true>(&bbmax[other]);
         mn = make_float4(fminf(mn.x, smn.x), fminf(mn.y, smn.y), fminf(mn.z, smn.z), 0.0f);
         mx = make_float4(fmaxf(mx.x, smx.x), fmaxf(mx.y, smx.y), fmaxf(mx.z, smx.z), 0.0f);
         area[cur] = box_area4(mn, mx);
         node = cur;
-        cur = parent[cur];
-        local = cur >= 0 && block_local(span[cur], cur, blk);
+        cur = npp;
+        if (cur >= 0) fetch(cur);
+        local = cur >= 0 && block_local(nsp, cur, blk);
         if (local || cur < 0) {
             bbmin[node] = mn;
             bbmax[node] = mx;

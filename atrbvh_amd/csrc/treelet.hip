@@ -646,20 +646,26 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
         int tc = 0;
         float my_sah = 0.0f;
         bool local = true;
+        int up = -1;
         if (cur >= 0) {
             // Nodes whose leaves all belong to this block meet in LDS; the rest
             // meet in global memory (cross-XCD hand-over, see device.hpp).  A
-            // node's own links and area are unchanged since the last launch
-            // until it is reached, so they are read plainly.
-            local = block_local(span[cur], cur, blk);
+            // node's own links, area, span and parent are unchanged from the
+            // start of the launch until the node is processed (only treelets
+            // rooted at it or above rewrite them), so they are read plainly and
+            // before the arrival: their latency overlaps the arrival's wait for
+            // this thread's earlier stores.
+            const int2 sp = span[cur];
+            const int l = left[cur];
+            const int r = right[cur];
+            const float ar = area[cur];
+            up = parent[cur];
+            local = block_local(sp, cur, blk);
             const uint32_t old =
                 local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
             if (old == 0u) {
                 cur = -1;
             } else {
-                const int l = left[cur];
-                const int r = right[cur];
-                const float ar = area[cur];
                 int tl, tr;
                 float sl, sr;
                 if (local) {
@@ -714,7 +720,7 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
                 }
             }
         }
-        if (cur >= 0) cur = parent[cur];
+        if (cur >= 0) cur = up;
     }
 }
 
