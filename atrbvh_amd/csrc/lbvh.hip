@@ -597,6 +597,14 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
     __shared__ uint8_t s_local[256];
     __shared__ uint64_t s_slot[256];
     __shared__ float s_box[256][2][6];
+    // Internal nodes made at block-local splits have indices inside this
+    // block's range; their links and far range end are staged here and
+    // written out, coalesced, when the block is done (scattered 4-byte stores
+    // cost a whole memory sector each).
+    __shared__ int32_t s_l[256];
+    __shared__ int32_t s_r[256];
+    __shared__ int32_t s_e[256];
+    __shared__ uint8_t s_staged[256];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -606,6 +614,7 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
     s_d[tid] = keys.delta(base - 1 + tid);
     if (tid == 0) s_d[256] = keys.delta(base + 255);
     s_slot[tid] = ~0ull;
+    s_staged[tid] = 0;
     __syncthreads();
     // inclusive prefix minimum of s_d[0..tid] and suffix minimum of s_d[tid+1..256]
     int pm = s_d[tid];
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
     __syncthreads();
 
     const uint32_t i = (uint32_t)base + (uint32_t)tid;
-    if (i >= n) return;
+    if (i < n) {
     const uint32_t tri = idx[i];
     const float4 a = v[3ull * tri];
     const float4 b = v[3ull * tri + 1];
@@ -712,14 +721,38 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
             dl = keys.delta(lo - 1);
         }
         const int p = (dl < 0 && dr < 0) ? lo : (dr > dl ? hi : lo);
-        left[p] = lchild;
-        right[p] = rchild;
-        data[p] = -1;
-        parent[lchild] = p;
-        parent[rchild] = p;
-        area[p] = box_area4(mn, mx);
-        span[p] = make_int2(lo, hi);
+        if (k >= 0 && k < 255 && s_local[k]) {
+            // p = lo or hi lies in this block's range; the rest at block end
+            s_l[p - base] = lchild;
+            s_r[p - base] = rchild;
+            s_e[p - base] = p == lo ? hi : lo;
+            s_staged[p - base] = 1;
+        } else {
+            left[p] = lchild;
+            right[p] = rchild;
+            data[p] = -1;
+            parent[lchild] = p;
+            parent[rchild] = p;
+            area[p] = box_area4(mn, mx);
+            span[p] = make_int2(lo, hi);
+        }
         node = p;
+    }
+    }
+    // Every staged node's box was stored by this block's thread that made it
+    // (before that thread's next split), so after the barrier its area is
+    // recomputed from the stored box, bit for bit the value in registers.
+    __syncthreads();
+    if (s_staged[tid]) {
+        const int x = base + tid;
+        const int l = s_l[tid], r = s_r[tid], e = s_e[tid];
+        left[x] = l;
+        right[x] = r;
+        data[x] = -1;
+        parent[l] = x;
+        parent[r] = x;
+        span[x] = make_int2(min(x, e), max(x, e));
+        area[x] = box_area4(bbmin[x], bbmax[x]);
     }
 }
 
