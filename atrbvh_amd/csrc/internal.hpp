@@ -57,6 +57,8 @@ struct atr_context {
     bool capturing = false;
     // device status word (bounded spin-wait overflow etc.)
     uint32_t* d_status = nullptr;
+    // compute units of the device (0 until first queried)
+    int cu_count = 0;
     // pinned host staging for small readbacks, handed out in fixed slots
     void* h_pinned = nullptr;
     std::vector<int> pinned_free;

@@ -159,6 +159,143 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ rays, 
     results[i] = make_int4(id, __float_as_int(t), __float_as_int(bu), __float_as_int(bv));
 }
 
+// Persistent traversal with dynamic ray fetch (the scheme of the reference's
+// kepler_dynamic_fetch kernel): each wave takes rays from a global counter in
+// batches as soon as a quarter of its lanes are idle, so lanes whose ray
+// missed early or finished early do not idle while the rest of the wave
+// traverses.  Per ray the traversal is exactly k_trace's (same slab test,
+// near child first, far child pushed, LIFO pops), so every ray gets the same
+// result.  The most recent stack entry lives in a register: a pop returns it
+// at once and the refill load from the private array overlaps the next node
+// fetch.  The host uses this kernel only when the tree height is below 64,
+// which bounds the stack.
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ rays, uint32_t R,
+                                                     const float4* __restrict__ nodes,
+                                                     const float4* __restrict__ tris,
+                                                     const int32_t* __restrict__ tri_id,
+                                                     int4* __restrict__ results,
+                                                     uint32_t* __restrict__ counter)
+{
+    constexpr int kStack = 64;
+    int stack[kStack];
+    const int lane = (int)__lane_id();
+    const uint64_t lt = lanemask_lt();
+    bool active = false;
+    bool exhausted = false;
+    uint32_t i = 0;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    float t = 0.0f, ix = 0.0f, iy = 0.0f, iz = 0.0f, bu = 0.0f, bv = 0.0f;
+    int id = -1, node = kDone, sp = 0, top = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        if (!exhausted && __popcll(idle) >= 16) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(idle));
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+            if (base >= R) exhausted = true;
+            if (!active) {
+                const uint32_t k = base + (uint32_t)__popcll(idle & lt);
+                if (k < R) {
+                    i = k;
+                    const float4 r0 = rays[2ull * i];
+                    const float4 r1 = rays[2ull * i + 1];
+                    ox = r0.x;
+                    oy = r0.y;
+                    oz = r0.z;
+                    tmin = r0.w;
+                    dx = r1.x;
+                    dy = r1.y;
+                    dz = r1.z;
+                    t = r1.w;
+                    ix = safe_inv(dx);
+                    iy = safe_inv(dy);
+                    iz = safe_inv(dz);
+                    id = -1;
+                    bu = 0.0f;
+                    bv = 0.0f;
+                    sp = 0;
+                    node = t > tmin ? 0 : kDone;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        // internal nodes until every active lane is at a leaf or done
+        while (__ballot(active && node >= 0) != 0ull) {
+            if (active && node >= 0) {
+                const float4 n0 = nodes[4ull * node + 0];
+                const float4 n1 = nodes[4ull * node + 1];
+                const float4 n2 = nodes[4ull * node + 2];
+                const float4 n3 = nodes[4ull * node + 3];
+                float t0, t1;
+                const bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, ox, oy, oz, ix, iy, iz,
+                                     tmin, t, t0);
+                const bool h1 = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, ox, oy, oz, ix, iy, iz,
+                                     tmin, t, t1);
+                const int c0 = __float_as_int(n3.x);
+                const int c1 = __float_as_int(n3.y);
+                if (h0 && h1) {
+                    const bool swap = t0 > t1;
+                    if (sp > 0) stack[sp - 1] = top;
+                    top = swap ? c0 : c1;
+                    ++sp;
+                    node = swap ? c1 : c0;
+                } else if (h0) {
+                    node = c0;
+                } else if (h1) {
+                    node = c1;
+                } else if (sp > 0) {
+                    node = top;
+                    --sp;
+                    if (sp > 0) top = stack[sp - 1];
+                } else {
+                    node = kDone;
+                }
+            }
+        }
+        if (active && node != kDone) {
+            // leaf: slots ~node, ~node+1, ... up to the one tagged last
+            bool stop = false;
+            for (int k = ~node;; ++k) {
+                const int tid = tri_id[k];
+                const float4 v0 = tris[3ull * k];
+                const float4 e1 = tris[3ull * k + 1];
+                const float4 e2 = tris[3ull * k + 2];
+                float tt, uu, vv;
+                if (intersect(v0, e1, e2, ox, oy, oz, dx, dy, dz, tmin, t, tt, uu, vv)) {
+                    t = tt;
+                    bu = uu;
+                    bv = vv;
+                    id = tid & 0x7fffffff;
+                    if (ANY) {
+                        stop = true;
+                        break;
+                    }
+                }
+                if (tid < 0) break;
+            }
+            if (stop || sp == 0) {
+                node = kDone;
+            } else {
+                node = top;
+                --sp;
+                if (sp > 0) top = stack[sp - 1];
+            }
+        }
+        if (active && node == kDone) {
+            results[i] = make_int4(id, __float_as_int(t), __float_as_int(bu), __float_as_int(bv));
+            active = false;
+        }
+    }
+}
+
+__global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+
 // Closest-hit traversal of a batch's binary trees (the forest): ray r belongs
 // to mesh r / rays_per_mesh and starts at that mesh's root (internal slot
 // seg[m]); leaves test their triangle (index into the batch's vertices).
@@ -374,14 +511,33 @@ atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uin
     int4* out = (int4*)results;
     const double bytes = (double)r * 48.0;
     if (depth < 64) {
-        return launch(ctx, any_hit ? "trace_any" : "trace", bytes, [&] {
-            if (any_hit)
-                k_trace<true, 64><<<grid, 256, 0, ctx->stream>>>(rays, r, c->tnodes, c->ttris,
-                                                                 c->ttri_id, out, nullptr, 0);
-            else
-                k_trace<false, 64><<<grid, 256, 0, ctx->stream>>>(rays, r, c->tnodes, c->ttris,
-                                                                  c->ttri_id, out, nullptr, 0);
+        // persistent waves: enough workgroups to fill every CU, at most one
+        // per 256 rays
+        if (ctx->cu_count == 0) {
+            int cus = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+                    hipSuccess ||
+                cus <= 0)
+                cus = 256;
+            ctx->cu_count = cus;
+        }
+        const uint32_t pgrid = std::min<uint32_t>(grid, (uint32_t)ctx->cu_count * 8u);
+        uint32_t* counter = nullptr;
+        ATR_TRY(dalloc_t(ctx, 1, &counter));
+        atr_status s = launch(ctx, "trace_zero", 4.0, [&] {
+            k_zero_word<<<1, 1, 0, ctx->stream>>>(counter);
         });
+        if (s == ATR_OK)
+            s = launch(ctx, any_hit ? "trace_any" : "trace", bytes, [&] {
+                if (any_hit)
+                    k_trace_fetch<true><<<pgrid, 256, 0, ctx->stream>>>(
+                        rays, r, c->tnodes, c->ttris, c->ttri_id, out, counter);
+                else
+                    k_trace_fetch<false><<<pgrid, 256, 0, ctx->stream>>>(
+                        rays, r, c->tnodes, c->ttris, c->ttri_id, out, counter);
+            });
+        dfree(ctx, counter);
+        return s;
     }
     int* gstack = nullptr;
     const int gdepth = depth + 1;
