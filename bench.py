@@ -56,13 +56,13 @@ def build_step(scene, treelet="atrbvh", want_sah=False):
 
 
 PMC_NAMES = {"build_refit63": "k_build<SingleKeys<unsigned long> >",
-             "onesweep64": "k_onesweep<unsigned long>", "encode63": "k_encode<unsigned long>"}
+             "onesweep64": "k_onesweep<unsigned long, 16>", "encode63": "k_encode<unsigned long>"}
 
 
 def traffic_from_profiles(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "c2_build_pmc_traffic.json")
+    path = os.path.join(ROOT, "profiles", "gmm_2m_build_pmc_traffic.json")
     try:
         with open(path) as f:
             k = json.load(f)["kernels"][PMC_NAMES[kernel]]
