@@ -55,10 +55,30 @@ __global__ __launch_bounds__(64) void k_forest_sah(const uint32_t* __restrict__ 
     if (m >= meshes) return;
     const uint32_t s = seg[m], e = seg[m + 1];
     const float total = area[s];
+    // Loads go out eight at a time ahead of the additions, which stay serial
+    // and in index order (the float sum is the reference's, bit for bit).
     float si = total;
-    for (uint32_t g = s + 1; g + 1 < e; ++g) si += area[g];
+    uint32_t g = s + 1;
+    const uint32_t ge = e - 1;  // internal nodes s+1 .. e-2
+    for (; g + 8 <= ge; g += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = area[g + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) si += v[k];
+    }
+    for (; g < ge; ++g) si += area[g];
     float sl = 0.0f;
-    for (uint32_t p = s; p < e; ++p) sl += area[first_leaf + p];
+    const float* la = area + first_leaf;
+    uint32_t p = s;
+    for (; p + 8 <= e; p += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = la[p + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sl += v[k];
+    }
+    for (; p < e; ++p) sl += la[p];
     out[m] = (ci * si + ct * sl) / total;
 }
 
