@@ -469,24 +469,45 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     const uint32_t local = block256_exclusive_sum(count, s_tmp, nullptr);
     s_local[d] = local;
 
+    // Look back over the predecessors' status words for this digit, nearest
+    // first, adding aggregates until an inclusive prefix: the words of up to
+    // kLookWindow tiles are loaded together and consumed in order up to the
+    // first one not yet published, which is then waited for.  Every
+    // published word is final for the value it carries, so the sum is the
+    // one-tile-at-a-time walk's.
+    constexpr int kLookWindow = 8;
     uint32_t excl = 0;
     if (tile > 0) {
         int t = (int)tile - 1;
         uint32_t spins = 0;
-        while (t >= 0) {
-            const uint32_t s = load_relaxed(&status[(size_t)t * 256u + d]);
-            const uint32_t flag = s & ~kValueMask;
-            if (flag == 0u) {
+        bool done = false;
+        while (!done) {
+            uint32_t w[kLookWindow];
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k)
+                w[k] = t - k >= 0 ? load_relaxed(&status[(size_t)(t - k) * 256u + d]) : 0u;
+            int used = 0;
+            bool stall = false;
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k) {
+                if (done || stall || t - k < 0) continue;
+                const uint32_t flag = w[k] & ~kValueMask;
+                if (flag == 0u) {
+                    stall = true;
+                    continue;
+                }
+                excl += w[k] & kValueMask;
+                ++used;
+                if (flag == kFlagInc) done = true;
+            }
+            t -= used;
+            if (!done && stall) {
                 if (++spins > kSpinLimit) {
                     atomicOr(err, 1u);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                continue;
             }
-            excl += s & kValueMask;
-            if (flag == kFlagInc) break;
-            --t;
         }
         store_relaxed(&st[d], kFlagInc | (excl + count));
     }
