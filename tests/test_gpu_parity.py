@@ -113,6 +113,57 @@ def test_atrbvh_bit_exact(ctx, treelet, iters):
     s.close()
 
 
+def _tie_scene(kind: str) -> np.ndarray:
+    """Scenes full of exact ties: `dup` repeats one 40-triangle soup 60 times
+    (equal Morton codes, boxes, areas and union areas); `grid` puts one
+    triangle shape on a 16 x 16 x 12 lattice (equal areas, and equal union
+    areas for every pair at the same lattice offset)."""
+    if kind == "dup":
+        base = ab.gen_soup(3, 40)
+        return np.ascontiguousarray(np.concatenate([base] * 60 + [ab.gen_soup(4, 37)]))
+    g = np.stack(np.meshgrid(np.arange(16), np.arange(16), np.arange(12), indexing="ij"), -1)
+    p = g.reshape(-1, 3).astype(np.float32) * np.float32(0.0625)
+    shape = np.array([[0.0, 0.0, 0.0], [0.03125, 0.0, 0.0], [0.0, 0.03125, 0.015625]],
+                     np.float32)
+    v = np.zeros((p.shape[0] * 3, 4), np.float32)
+    v[:, :3] = (p[:, None, :] + shape[None, :, :]).reshape(-1, 3)
+    return v
+
+
+@pytest.mark.parametrize("kind", ["dup", "grid"])
+@pytest.mark.parametrize("opt", ["atrbvh9", "atrbvh5", "atrbvh16", "trbvh"])
+def test_treelets_and_collapse_under_ties(ctx, kind, opt):
+    """Treelet restructuring and collapse where many distances, areas and
+    keys are exactly equal, so every tie rule of the reference's reductions
+    (findLeafToExpand, findMinimum, reduceOptimal) decides the tree."""
+    v = _tie_scene(kind)
+    n = v.shape[0] // 3
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(False).BuildTree(s)
+    ref = oracle.lbvh(v, mn, mx, 30)
+    if opt == "trbvh":
+        ab.TRBVHOptimizer(7, 3).Optimize(t)
+        oracle.trbvh(ref, 7, 3)
+    else:
+        size = int(opt[6:])
+        ab.AgglomerativeTreeletOptimizer(size, 2).Optimize(t)
+        oracle.atrbvh(ref, size, 2)
+    dev = t.download()
+    assert_tree_equal(dev, tree_arrays(ref), f"{kind}-{opt}")
+    check_tree_invariants(dev, n)
+    c, sah = ab.BVHTreeCollapser().Collapse(t)
+    rc, rsah = oracle.collapse(ref)
+    cd = c.download()
+    assert bits(np.float32(sah)) == bits(np.float32(rsah))
+    for k in ("parent", "left", "right", "data", "count"):
+        assert np.array_equal(cd[k], getattr(rc, k)), k
+    assert cd["used"] == rc.used
+    assert np.array_equal(cd["tri_index"][: rc.used], rc.tri_index[: rc.used])
+    c.close()
+    t.close()
+    s.close()
+
+
 @pytest.mark.parametrize("bits_,n", [(30, C1_N), (63, 30000), (30, 9), (30, 300)])
 def test_trbvh_bit_exact(ctx, bits_, n):
     v = ab.gen_soup(C1_SEED if n == C1_N else 77 + n, n)
