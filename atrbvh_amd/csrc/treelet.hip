@@ -8,10 +8,17 @@
 // A wave of 64 lanes takes the voting treelet roots two at a time, one per
 // 32-lane half.  Treelets voted in the same round are disjoint subtrees, so
 // processing two concurrently equals the reference's one-at-a-time loop.
-// Inside a half every cross-lane reduction is a width-32 __shfl_down halving
-// tree with the reference's widths and strict comparisons, so ties resolve to
-// the same lane (findLeafToExpand Treelet.cuh:23-39, findMinimum
-// AgglomerativeTreelet.cu:65-81, reduceOptimal TRBVH.cu:23-35).
+// Inside a half every cross-lane reduction resolves ties exactly as the
+// reference's width-32 __shfl_down halving trees with strict comparisons do
+// (findLeafToExpand Treelet.cuh:23-39, findMinimum AgglomerativeTreelet.cu:
+// 65-81, reduceOptimal TRBVH.cu:23-35): findLeafToExpand and reduceOptimal
+// run the same tree; findMinimum takes the minimum value with DPP row shifts
+// and picks, from a ballot of the lanes holding it, the one with the
+// smallest 5-bit-reversed lane index -- the lane that tree selects.
+//
+// ATRBVH forms treelets from 16-byte node records {left, right,
+// area(left), area(right)} that k_make_records builds before the first pass
+// and every treelet rewrite keeps current.
 #include "device.hpp"
 #include "internal.hpp"
 
