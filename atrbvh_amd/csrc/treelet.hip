@@ -707,16 +707,21 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
             const float root_sah = __shfl(my_sah, src, 64);
             const int root_tris = __shfl(tc, src, 64);
             const bool root_local = __shfl((int)local, src, 64) != 0;
+            // The two halves share one instruction stream: when either root
+            // is crossing, both treelets use the device-coherent loads (which
+            // are also correct for a block-local treelet), so the wave runs
+            // one flavour instead of both in turn.
+            const bool wave_local = __ballot(my_r >= 0 && !root_local) == 0ull;
             if (my_r >= 0) {
                 if (AGG) {
-                    if (root_local)
+                    if (wave_local)
                         atrbvh_treelet<false>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
                                               right, bbmin, bbmax, area, nsah, span, rec);
                     else
                         atrbvh_treelet<true>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
                                              right, bbmin, bbmax, area, nsah, span, rec);
                 } else {
-                    if (root_local)
+                    if (wave_local)
                         trbvh_treelet<false>(lds, hl, root, root_sah, root_tris, first_leaf, ci, ct, sched,
                                              rounds, parent, left, right, bbmin, bbmax, area,
                                              nsah, span);
