@@ -708,7 +708,7 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
         // highest-lane root waits one step if other lanes are still climbing
         // (they may bring a partner), it has not waited before and its
         // subtree is small (the long chain near the root is not delayed).
-        // Only the TRBVH pass does this: it shortens that pass by ~3%, while
+        // Only the TRBVH pass does this: it shortens that pass by ~2%, while
         // the cheaper agglomerative treelets measured ~3% slower with it.
         if (!AGG && (__popcll(vote) & 1) != 0) {
             const int last = 63 - (int)__clzll((unsigned long long)vote);
