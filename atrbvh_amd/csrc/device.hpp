@@ -238,10 +238,11 @@ __device__ __forceinline__ uint64_t exchange_crossing(uint64_t* slot, uint64_t v
 }
 
 // Block-local test: every leaf under the node (span) and the node index itself
-// lie in this workgroup's 256-leaf block.
+// lie in this workgroup's block of 2^SHIFT leaves (256 unless stated).
+template <int SHIFT = 8>
 __device__ __forceinline__ bool block_local(int2 span, int node, int blk)
 {
-    return (span.x >> 8) == blk && (span.y >> 8) == blk && (node >> 8) == blk;
+    return (span.x >> SHIFT) == blk && (span.y >> SHIFT) == blk && (node >> SHIFT) == blk;
 }
 
 // --------------------------------------------------------------- wave ops

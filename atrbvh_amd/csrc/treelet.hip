@@ -619,8 +619,14 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
 }
 
 // ------------------------------------------------------------------ driver
+// Workgroups of 128 threads (one leaf each, two waves): smaller groups free
+// their LDS sooner once most of their threads have stopped climbing, which
+// measured ~2% faster than 256 on the 2M-triangle build (64 was slower).
+constexpr int kPassBlock = 128;
+constexpr int kPassShift = 7;
+
 template <bool AGG>
-__global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int first_leaf, int n,
+__global__ __launch_bounds__(kPassBlock) void k_treelet_pass(uint32_t n_leaves, int first_leaf, int n,
                                                       int gamma, float ci,
                                                       float ct, int32_t* parent, int32_t* left,
                                                       int32_t* right, float4* bbmin,
@@ -631,14 +637,14 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
                                                       int half_words)
 {
     extern __shared__ float smem[];
-    __shared__ uint32_t s_cnt[256];
+    __shared__ uint32_t s_cnt[kPassBlock];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int half = lane >> 5;
     const int hl = lane & 31;
     float* lds = smem + ((tid >> 6) * 2 + half) * half_words;
 
-    const uint32_t gid = blockIdx.x * 256u + (uint32_t)tid;
+    const uint32_t gid = blockIdx.x * (uint32_t)kPassBlock + (uint32_t)tid;
     const int blk = (int)blockIdx.x;
     s_cnt[tid] = 0u;
     __syncthreads();
@@ -677,9 +683,10 @@ __global__ __launch_bounds__(256) void k_treelet_pass(uint32_t n_leaves, int fir
             const int r = right[cur];
             const float ar = area[cur];
             up = parent[cur];
-            local = block_local(sp, cur, blk);
+            local = block_local<kPassShift>(sp, cur, blk);
             const uint32_t old =
-                local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
+                local ? arrive_local(&s_cnt[cur & (kPassBlock - 1)])
+                      : arrive_crossing(&counters[cur]);
             if (old == 0u) {
                 cur = -1;
             } else {
@@ -817,9 +824,9 @@ atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma
     const int first_leaf = (int)t->first_leaf;
     ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
     const int half_words = kAggFixedWords + treelet * (treelet - 1) / 2;
-    const size_t lds = (size_t)half_words * 8 * sizeof(float);
+    const size_t lds = (size_t)half_words * (kPassBlock / 32) * sizeof(float);
     return launch(ctx, "atrbvh_pass", 0.0, [&] {
-        k_treelet_pass<true><<<div_up(N, 256), 256, lds, ctx->stream>>>(
+        k_treelet_pass<true><<<div_up(N, kPassBlock), kPassBlock, lds, ctx->stream>>>(
             N, first_leaf, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin,
             t->bbmax, t->area, tris, nsah, counters, t->span, rec, nullptr, 0, half_words);
     });
@@ -832,9 +839,9 @@ atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, fl
     const uint32_t N = t->n;
     const int first_leaf = (int)t->first_leaf;
     ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
-    const size_t lds = (size_t)kTrWords * 8 * sizeof(float);
+    const size_t lds = (size_t)kTrWords * (kPassBlock / 32) * sizeof(float);
     return launch(ctx, "trbvh_pass", 0.0, [&] {
-        k_treelet_pass<false><<<div_up(N, 256), 256, lds, ctx->stream>>>(
+        k_treelet_pass<false><<<div_up(N, kPassBlock), kPassBlock, lds, ctx->stream>>>(
             N, first_leaf, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax,
             t->area, tris, nsah, counters, t->span, nullptr, d_schedule, rounds, kTrWords);
     });
