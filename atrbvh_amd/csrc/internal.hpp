@@ -89,8 +89,9 @@ struct atr_tree {
     float4* bbmin = nullptr;
     float4* bbmax = nullptr;
     float* area = nullptr;
-    // [min, max] leaf position (leaf node index - (n-1)) under each node; a
-    // node whose span lies in one 256-leaf block is handled by that block only
+    // [min, max] leaf position (leaf node index - first_leaf) under each
+    // node; a node whose span and index lie in one workgroup's block of leaves
+    // (256, or 128 in the treelet passes) is handled by that workgroup only
     int2* span = nullptr;
 };
 

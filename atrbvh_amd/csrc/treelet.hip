@@ -621,7 +621,7 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
 // ------------------------------------------------------------------ driver
 // Workgroups of 128 threads (one leaf each, two waves): smaller groups free
 // their LDS sooner once most of their threads have stopped climbing, which
-// measured ~2% faster than 256 on the 2M-triangle build (64 was slower).
+// measured ~3% faster than 256 on the 2M-triangle build (64 was slower).
 constexpr int kPassBlock = 128;
 constexpr int kPassShift = 7;
 
