@@ -61,19 +61,9 @@ __device__ __forceinline__ uint64_t spread21(uint64_t v)
 }
 
 // --------------------------------------------------------------- atomics
-// Agent-scope (whole GPU, all XCDs) orderings.  On gfx950 an acq_rel RMW
-// brackets the atomic with an L2 writeback (release) and invalidate
-// (acquire), which the cross-XCD bottom-up protocols need.
-__device__ __forceinline__ uint32_t atomic_add_acq_rel(uint32_t* p, uint32_t v)
-{
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint64_t atomic_xchg_acq_rel(uint64_t* p, uint64_t v)
-{
-    return __hip_atomic_exchange(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// Agent-scope (whole GPU, all XCDs) relaxed operations.  (An acq_rel RMW at
+// this scope would bracket the atomic with an L2 writeback and invalidate on
+// gfx950; the bottom-up protocols below avoid that, see the hand-over notes.)
 __device__ __forceinline__ uint32_t load_relaxed(const uint32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
