@@ -158,6 +158,35 @@ def test_treelets_and_collapse_under_ties(ctx, kind, opt):
     for k in ("parent", "left", "right", "data", "count"):
         assert np.array_equal(cd[k], getattr(rc, k)), k
     assert cd["used"] == rc.used
+
+
+@pytest.mark.parametrize("kind", ["dup", "grid"])
+def test_63bit_build_and_atrbvh_under_ties(ctx, kind):
+    """The 63-bit Morton path (the benchmark's) on the same tie scenes: equal
+    64-bit keys fall back to the index prefix in the split search, the sort
+    must keep equal keys in index order, and ATRBVH (9, 2) + collapse must
+    then match the oracle bit for bit."""
+    v = _tie_scene(kind)
+    n = v.shape[0] // 3
+    s, mn, mx = _scene_pair(ctx, v)
+    t = ab.LBVHBuilder(True).BuildTree(s)
+    ref = oracle.lbvh(v, mn, mx, 63)
+    assert_tree_equal(t.download(), tree_arrays(ref), f"{kind}-lbvh63")
+    ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(t)
+    oracle.atrbvh(ref, 9, 2)
+    dev = t.download()
+    assert_tree_equal(dev, tree_arrays(ref), f"{kind}-lbvh63-atrbvh9")
+    check_tree_invariants(dev, n)
+    c, sah = ab.BVHTreeCollapser().Collapse(t)
+    rc, rsah = oracle.collapse(ref)
+    cd = c.download()
+    assert bits(np.float32(sah)) == bits(np.float32(rsah))
+    for k in ("parent", "left", "right", "data", "count"):
+        assert np.array_equal(cd[k], getattr(rc, k)), k
+    assert cd["used"] == rc.used
+    c.close()
+    t.close()
+    s.close()
     assert np.array_equal(cd["tri_index"][: rc.used], rc.tri_index[: rc.used])
     c.close()
     t.close()
