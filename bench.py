@@ -45,8 +45,8 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_step(scene, treelet="atrbvh", want_sah=False):
-    tree = ab.LBVHBuilder(True).BuildTree(scene)
+def build_step(scene, treelet="atrbvh", want_sah=False, morton63=True):
+    tree = ab.LBVHBuilder(morton63).BuildTree(scene)
     if treelet == "atrbvh":
         ab.AgglomerativeTreeletOptimizer(9, 2).Optimize(tree)
     elif treelet == "trbvh":
@@ -376,6 +376,60 @@ def copy_calibration(nbytes: int = 256 << 20):
     log(f"[bench] copy calibration: 3 copies of {nbytes} bytes")
 
 
+def scaling_leg(reps: int = 10):
+    """LBVH + ATRBVH (9, 2) + collapse at several sizes: the 65,536-triangle
+    soup with 30-bit keys and its 256 x 256 pinhole rays (the reference's own
+    CPU-runnable case), and the GMM scene family of the headline build with
+    63-bit keys at 262,144 to 4,194,304 triangles.  Prints one JSON line of
+    rows."""
+    ctx = ab.Context(0)
+    rows = []
+    cases = [("soup", 65_536, False), ("gmm", 262_144, True), ("gmm", 1_048_576, True),
+             ("gmm", 2_097_152, True), ("gmm", 4_194_304, True)]
+    for kind, n, m63 in cases:
+        v = ab.gen_soup(1, n) if kind == "soup" else ab.gen_gmm(SEED, n)
+        scene = ctx.scene(v)
+        for _ in range(2):
+            t, c, _ = build_step(scene, morton63=m63)
+            c.close()
+            t.close()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            t, c, _ = build_step(scene, morton63=m63)
+            c.close()
+            t.close()
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        t, c, sah = build_step(scene, want_sah=True, morton63=m63)
+        row = {"scene": kind, "triangles": n, "morton_bits": 63 if m63 else 30,
+               "ms_per_build": round(ms, 4), "Mtris_s": round(n / (ms * 1e-3) / 1e6, 2),
+               "sah": round(float(sah), 4)}
+        if kind == "soup":
+            g = 256
+            d_rays = ctx.buffer(g * g * 32)
+            d_res = ctx.buffer(g * g * 16)
+            ctx.rays_pinhole((0.5, 0.5, -1.0), (1, 0, 0, 0, 1, 0, 0, 0, 1), 1.0, g, g, np.inf,
+                             d_rays)
+            c.trace(d_rays, g * g, d_res)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                c.trace(d_rays, g * g, d_res)
+            ctx.sync()
+            tms = (time.perf_counter() - t0) * 1e3 / reps
+            row["trace_ms"] = round(tms, 4)
+            row["trace_Mrays_s"] = round(g * g / (tms * 1e-3) / 1e6, 1)
+        c.close()
+        t.close()
+        scene.close()
+        log(f"[scaling] {kind} {n:,} ({row['morton_bits']}-bit): {ms:.3f} ms, "
+            f"{row['Mtris_s']} Mtris/s, SAH {row['sah']}")
+        rows.append(row)
+    print(json.dumps({"scaling": rows, "workload": "LBVH + ATRBVH(9,2) + collapse per size",
+                      "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,12 +443,17 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--no-batch-adjoint", action="store_true",
                     help="skip the batched differentiable leg")
+    ap.add_argument("--scaling", action="store_true",
+                    help="only time the build at several sizes (one JSON line of rows)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.scaling:
+        scaling_leg()
         return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU "
