@@ -101,6 +101,7 @@ def lib():
         L.atr_tree_download.argtypes = [_vp, ctypes.POINTER(TreeView)]
         L.atr_tree_device_view.argtypes = [_vp, ctypes.POINTER(TreeView)]
         L.atr_tree_upload.argtypes = [_vp, ctypes.POINTER(TreeView)]
+        L.atr_tree_view_check.argtypes = [ctypes.POINTER(TreeView)]
         L.atr_tree_create_from_host.argtypes = [_vp, ctypes.POINTER(TreeView),
                                                 ctypes.POINTER(_vp)]
         L.atr_tree_bind_scene.argtypes = [_vp, _vp]

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -148,6 +149,12 @@ atr_status atr_context_create(int device, atr_context** out)
     ATR_HIP(hipSetDevice(device));
     atr_context* ctx = new atr_context();
     ctx->device = device;
+    ctx->spin_limit = kSpinLimit;
+    if (const char* lim = std::getenv("ATRBVH_SPIN_LIMIT")) {
+        char* end = nullptr;
+        const unsigned long v = std::strtoul(lim, &end, 10);
+        if (end != lim && *end == '\0' && v <= kSpinLimit) ctx->spin_limit = (uint32_t)v;
+    }
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx;
@@ -185,9 +192,16 @@ atr_status atr_sync(atr_context* ctx)
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     uint32_t st = 0;
     ATR_HIP(hipMemcpy(&st, ctx->d_status, 4, hipMemcpyDeviceToHost));
-    if (st & 1u) return fail(ATR_ERR_INTERNAL, "a bounded device wait gave up");
-    if (st & 2u) return fail(ATR_ERR_UNSUPPORTED, "a tree was deeper than the traversal stack");
-    return ATR_OK;
+    if (st == 0u) return ATR_OK;
+    // Reported once: the word is cleared so that later work on this context
+    // is judged on its own.
+    ATR_HIP(hipMemsetAsync(ctx->d_status, 0, 4, ctx->stream));
+    ATR_HIP(hipStreamSynchronize(ctx->stream));
+    if (st & 1u)
+        return fail(ATR_ERR_INTERNAL,
+                    "a bounded device wait gave up (the sort's results were discarded and "
+                    "trees built since the last synchronisation use input order)");
+    return fail(ATR_ERR_UNSUPPORTED, "a tree was deeper than the traversal stack");
 }
 
 void* atr_context_stream(atr_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
@@ -588,6 +602,66 @@ atr_status atr_tree_download(atr_tree* t, atr_tree_view* h)
     return ATR_OK;
 }
 
+// Every kernel that walks a tree trusts its links, so a tree handed in by a
+// caller is checked in full on the host: the BVHRT index convention (root 0,
+// internal nodes 0..n-2 with data -1, leaves n-1..2n-2 with no children and
+// data a permutation of 0..n-1), every node but the root the child of exactly
+// one internal node, parent[] equal to those links, and every node reachable
+// from the root (which rules out cycles among internal nodes).
+atr_status atr_tree_view_check(const atr_tree_view* h)
+{
+    if (!h || !h->parent || !h->left || !h->right || !h->data)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    const uint32_t n = h->num_triangles;
+    if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a tree needs at least 2 triangles");
+    if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 triangles per tree");
+    if (h->root != 0) return fail(ATR_ERR_INVALID_ARGUMENT, "the root must be node 0");
+    const size_t m = 2ull * n - 1;
+    std::vector<uint8_t> seen(m, 0);
+    for (uint32_t p = 0; p + 1 < n; ++p) {
+        if (h->data[p] != -1)
+            return fail(ATR_ERR_INVALID_ARGUMENT,
+                        "internal node " + std::to_string(p) + " has data != -1");
+        for (int32_t ch : {h->left[p], h->right[p]}) {
+            if (ch <= 0 || (size_t)ch >= m || seen[ch])
+                return fail(ATR_ERR_INVALID_ARGUMENT,
+                            "tree view is not a binary tree (node " + std::to_string(p) + ")");
+            seen[ch] = 1;
+            if (h->parent[ch] != (int32_t)p)
+                return fail(ATR_ERR_INVALID_ARGUMENT,
+                            "parent[" + std::to_string(ch) + "] does not match the links");
+        }
+    }
+    if (h->parent[0] != -1) return fail(ATR_ERR_INVALID_ARGUMENT, "parent[root] must be -1");
+    std::vector<uint8_t> tri(n, 0);
+    for (size_t i = n - 1; i < m; ++i) {
+        if (h->left[i] != -1 || h->right[i] != -1)
+            return fail(ATR_ERR_INVALID_ARGUMENT, "leaf " + std::to_string(i) + " has children");
+        const int32_t d = h->data[i];
+        if (d < 0 || (uint32_t)d >= n || tri[d])
+            return fail(ATR_ERR_INVALID_ARGUMENT,
+                        "leaf data must be a permutation of the triangle indices");
+        tri[d] = 1;
+    }
+    // reachability from the root: each node has one parent, so a walk visits
+    // every reachable node once
+    std::vector<int32_t> stack{0};
+    size_t visited = 0;
+    while (!stack.empty()) {
+        const int32_t x = stack.back();
+        stack.pop_back();
+        ++visited;
+        if ((uint32_t)x + 1 < n) {
+            stack.push_back(h->left[x]);
+            stack.push_back(h->right[x]);
+        }
+    }
+    if (visited != m)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "tree view has nodes unreachable from the root (a cycle)");
+    return ATR_OK;
+}
+
 atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
 {
     if (!t || !h || !h->parent || !h->left || !h->right || !h->data || !h->bbmin || !h->bbmax ||
@@ -595,20 +669,7 @@ atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (h->num_triangles != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "size mismatch");
     const size_t m = 2ull * t->n - 1;
-    // topology check: every node but the root is somebody's child exactly once
-    {
-        std::vector<uint8_t> seen(m, 0);
-        for (uint32_t p = 0; p + 1 < t->n; ++p) {
-            for (int32_t ch : {h->left[p], h->right[p]}) {
-                if (ch <= 0 || (size_t)ch >= m || seen[ch])
-                    return fail(ATR_ERR_INVALID_ARGUMENT, "tree view is not a binary tree");
-                seen[ch] = 1;
-            }
-        }
-        for (size_t i = t->n - 1; i < m; ++i)
-            if (h->data[i] < 0 || (uint32_t)h->data[i] >= t->n)
-                return fail(ATR_ERR_INVALID_ARGUMENT, "leaf data index out of range");
-    }
+    ATR_TRY(atr_tree_view_check(h));
     ATR_HIP(hipStreamSynchronize(t->ctx->stream));
     ATR_HIP(hipMemcpy(t->parent, h->parent, m * 4, hipMemcpyHostToDevice));
     ATR_HIP(hipMemcpy(t->left, h->left, m * 4, hipMemcpyHostToDevice));
@@ -1008,6 +1069,7 @@ static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_
                                ctx->stream);
         if (e == hipSuccess && bboxes) {
             b->bbox_given = true;
+            b->h_bbox.assign(bboxes, bboxes + 6ull * meshes);
             e = hipMemcpyAsync(b->d_bbox, bboxes, 6ull * meshes * 4, hipMemcpyHostToDevice,
                                ctx->stream);
         }
@@ -1061,7 +1123,7 @@ static atr_status batch_enqueue(atr_batch* b, int treelet, int iterations, float
                      b->sort_items));
     const int k = in_alt ? 1 : 0;
     ATR_TRY(run_build_forest(ctx, b->keys[k], b->vals[k], b->verts, b->forest, b->d_seg,
-                             b->slots));
+                             b->meshes, b->slots));
     int gamma = treelet;
     if (iterations > 0) ATR_TRY(run_make_records(ctx, b->forest, b->rec));
     for (int it = 0; it < iterations; ++it) {
@@ -1236,7 +1298,18 @@ atr_status atr_batch_rebuild(atr_batch* b, const uint32_t* meshes, uint32_t coun
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "sub-batch map upload failed");
     }
-    if (s == ATR_OK) s = batch_create(ctx, (const float*)sv, true, off.data(), count, nullptr, &sb);
+    // A batch built on caller boxes keeps them: the rebuilt meshes quantise
+    // their Morton codes in the same frames as a full build does.
+    std::vector<float> boxes;
+    if (b->bbox_given) {
+        boxes.reserve(6ull * count);
+        for (uint32_t j = 0; j < count; ++j)
+            boxes.insert(boxes.end(), b->h_bbox.begin() + 6ull * meshes[j],
+                         b->h_bbox.begin() + 6ull * meshes[j] + 6);
+    }
+    if (s == ATR_OK)
+        s = batch_create(ctx, (const float*)sv, true, off.data(), count,
+                         b->bbox_given ? boxes.data() : nullptr, &sb);
     if (s == ATR_OK) s = atr_batch_build(sb, treelet_size, iterations, ci, ct, 0);
     if (s == ATR_OK)
         s = run_forest_splice(ctx, sb->forest, sb->d_seg, count, d_map, b->d_seg, b->forest);

@@ -272,8 +272,4 @@ __device__ __forceinline__ uint32_t block256_exclusive_sum(uint32_t v, uint32_t*
     return add + inc - v;
 }
 
-// Maximum spin iterations of a bounded wait before it gives up and flags an
-// error (a correct launch never gets close).
-constexpr uint32_t kSpinLimit = 1u << 26;
-
 }  // namespace atr

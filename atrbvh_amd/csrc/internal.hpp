@@ -12,6 +12,10 @@
 
 namespace atr {
 
+// Default number of spin iterations of a bounded wait before it gives up and
+// flags an error (a correct launch never gets close).
+constexpr uint32_t kSpinLimit = 1u << 26;
+
 // ------------------------------------------------------------------ errors
 void set_error(const std::string& msg);
 atr_status fail(atr_status code, const std::string& msg);
@@ -55,8 +59,12 @@ struct atr_context {
     // true while the stream is being captured into a HIP graph (no profiling
     // events are recorded then)
     bool capturing = false;
-    // device status word (bounded spin-wait overflow etc.)
+    // device status word: bit 0 = a bounded wait gave up, bit 1 = a tree was
+    // deeper than a traversal stack; cleared when atr_sync reports it
     uint32_t* d_status = nullptr;
+    // iterations of a bounded wait before it gives up (ATRBVH_SPIN_LIMIT
+    // overrides the default at context creation)
+    uint32_t spin_limit = 0;
     // compute units of the device (0 until first queried)
     int cu_count = 0;
     // pinned host staging for small readbacks, handed out in fixed slots
@@ -136,6 +144,7 @@ struct atr_batch {
     float4* verts = nullptr;       // 3n, owned
     float* d_bbox = nullptr;       // 6 per mesh
     bool bbox_given = false;
+    std::vector<float> h_bbox;     // caller's boxes (6 per mesh) when given
     uint32_t* bbox_acc = nullptr;  // 6 per mesh
     atr_tree* forest = nullptr;    // first_leaf = n, 2n node slots
     // persistent workspace (every build, eager or graph, uses these)
@@ -206,7 +215,12 @@ atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const f
 struct SortWork {
     uint32_t* offsets;
     uint32_t* status;
+    // digit histograms when the caller has none (hist_words(n, passes) words;
+    // may be NULL when block histograms are always given)
+    uint32_t* hist = nullptr;
 };
+// Words of the per-block digit histograms of a sort of n keys.
+size_t sort_hist_words(uint32_t n, int passes);
 size_t sort_status_words(uint32_t n, int passes);
 atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint32_t* vals,
                     uint32_t n, void* keys_alt, uint32_t* vals_alt, bool vals_identity,
@@ -223,7 +237,7 @@ atr_status run_encode_forest(atr_context* ctx, const float4* verts, const uint32
                              int items);
 atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32_t* idx,
                             const float4* verts, atr_tree* t, const uint32_t* d_seg,
-                            uint64_t* slots);
+                            uint32_t meshes, uint64_t* slots);
 uint32_t encode_blocks(uint32_t n);
 atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
                      uint32_t n, const float4* verts, atr_tree* t, uint64_t* slots);

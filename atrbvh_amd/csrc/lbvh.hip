@@ -397,7 +397,7 @@ template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ digit_off,
-    uint32_t* status, uint32_t* tile_ticket, uint32_t* err)
+    uint32_t* status, uint32_t* tile_ticket, uint32_t* err, uint32_t spin_limit)
 {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_hist[kSortWaves][256];
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
             }
             t -= used;
             if (!done && stall) {
-                if (++spins > kSpinLimit) {
+                if (++spins > spin_limit) {
                     atomicOr(err, 1u);
                     break;
                 }
@@ -571,13 +571,23 @@ __device__ __forceinline__ int delta_at(const uint64_t* k, int i)
 // Sorted-key views for the build.  delta(i) is the split strength between
 // sorted positions i and i+1; -1 marks a wall that no node spans (the ends of
 // the array, and in a forest the boundary between two meshes).
+//
+// Both views have a fallback form (`fb`), used when a bounded wait of the
+// sort that produced the keys gave up (bit 0 of the context's status word):
+// the sorted keys and indices may then be incomplete, so the build ignores
+// them and makes the radix tree of the triangles in input order (every key
+// taken as equal, so the split strengths come from positions alone).  The
+// tree is well formed, so every later kernel stays in bounds, and the error
+// is reported at the next synchronisation.
 template <typename K>
 struct SingleKeys {
     const K* k;
     int last;  // n - 1
+    bool fb = false;
     __device__ __forceinline__ int delta(int i) const
     {
-        return (i < 0 || i >= last) ? -1 : delta_at(k, i);
+        if (i < 0 || i >= last) return -1;
+        return fb ? 32 + __clz((uint32_t)(i ^ (i + 1))) : delta_at(k, i);
     }
 };
 
@@ -588,10 +598,24 @@ struct SingleKeys {
 struct ForestKeys {
     const uint64_t* k;
     int last;
-    const uint32_t* seg_start;  // first sorted position of every mesh
+    const uint32_t* seg_start;  // first sorted position of every mesh (meshes + 1 entries)
+    uint32_t meshes;
+    bool fb = false;
     __device__ __forceinline__ int delta(int i) const
     {
         if (i < 0 || i >= last) return -1;
+        if (fb) {
+            // mesh m holds positions seg_start[m] .. seg_start[m+1]-1
+            uint32_t lo = 0, hi = meshes;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((int)seg_start[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            if ((int)seg_start[lo + 1] == i + 1) return -1;
+            const int s = (int)seg_start[lo];
+            return 32 + __clz((uint32_t)((i - s) ^ (i + 1 - s)));
+        }
         const uint64_t a = k[i], b = k[i + 1];
         if ((a ^ b) >> 30) return -1;
         const uint32_t x = (uint32_t)a ^ (uint32_t)b;
@@ -606,13 +630,17 @@ struct ForestKeys {
 // walls) at lo -- position 0 for a single tree, the mesh's first position in a
 // forest.  Leaf of sorted position i is node first_leaf + i.
 template <typename KV>
-__global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void k_build(KV kv_in, const uint32_t* __restrict__ idx,
                                                uint32_t n, int first_leaf,
                                                const float4* __restrict__ v, int32_t* parent,
                                                int32_t* left, int32_t* right, int32_t* data,
                                                float4* bbmin, float4* bbmax, float* area,
-                                               int2* span, uint64_t* slots)
+                                               int2* span, uint64_t* slots,
+                                               const uint32_t* __restrict__ status)
 {
+    // bit 0 of the status word: a bounded wait of the sort gave up
+    KV keys = kv_in;
+    keys.fb = (status[0] & 1u) != 0u;
     __shared__ int s_d[257];
     __shared__ int s_wmin[8];
     __shared__ uint8_t s_local[256];
@@ -670,7 +698,7 @@ __global__ __launch_bounds__(256) void k_build(KV keys, const uint32_t* __restri
 
     const uint32_t i = (uint32_t)base + (uint32_t)tid;
     if (i < n) {
-    const uint32_t tri = idx[i];
+    const uint32_t tri = keys.fb ? i : idx[i];
     const float4 a = v[3ull * tri];
     const float4 b = v[3ull * tri + 1];
     const float4 c = v[3ull * tri + 2];
@@ -902,6 +930,11 @@ size_t sort_status_words(uint32_t n, int passes)
     return (size_t)passes * (std::max(div_up(n, (uint64_t)kSortBlock * 8), 1u) * 256u + 64u);
 }
 
+size_t sort_hist_words(uint32_t n, int passes)
+{
+    return (size_t)std::max(div_up(n, kEncTile), 1u) * passes * 256u;
+}
+
 // Stable LSD sort over `passes` 8-bit digits (the low 8*passes key bits).
 // Results alternate between (keys, vals) and (keys_alt, vals_alt);
 // *result_in_alt says where they end.  block_hist may come from the encode
@@ -926,21 +959,27 @@ atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint
         ATR_TRY(dalloc_t(ctx, (size_t)passes * 256, &offsets));
         ATR_TRY(dalloc_t(ctx, status_words, &status));
     }
+    uint32_t* hist_buf = nullptr;
     if (!block_hist) {
         hist_blocks = std::max(div_up(n, kEncTile), 1u);
-        ATR_TRY(dalloc_t(ctx, (size_t)hist_blocks * passes * 256, &own_hist));
+        if (work && work->hist) {
+            hist_buf = work->hist;
+        } else {
+            ATR_TRY(dalloc_t(ctx, (size_t)hist_blocks * passes * 256, &own_hist));
+            hist_buf = own_hist;
+        }
         if (key_bits == 64) {
             ATR_TRY(launch(ctx, "histogram64", (double)n * kbytes, [&] {
                 k_histogram<uint64_t><<<hist_blocks, 256, 0, ctx->stream>>>(
-                    (const uint64_t*)keys, n, passes, own_hist);
+                    (const uint64_t*)keys, n, passes, hist_buf);
             }));
         } else {
             ATR_TRY(launch(ctx, "histogram32", (double)n * kbytes, [&] {
                 k_histogram<uint32_t><<<hist_blocks, 256, 0, ctx->stream>>>(
-                    (const uint32_t*)keys, n, passes, own_hist);
+                    (const uint32_t*)keys, n, passes, hist_buf);
             }));
         }
-        block_hist = own_hist;
+        block_hist = hist_buf;
     }
     const uint64_t offset_words = (uint64_t)passes * 256;
     ATR_TRY(launch(ctx, "fill_status", (double)(status_words + offset_words) * 4.0, [&] {
@@ -968,22 +1007,22 @@ atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint
                 if (tile_items == 16)
                     k_onesweep<uint64_t, 16><<<tiles, kSortBlock, 0, ctx->stream>>>(
                         (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur],
-                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
                 else
                     k_onesweep<uint64_t, 8><<<tiles, kSortBlock, 0, ctx->stream>>>(
                         (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur],
-                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
             });
         } else {
             s = launch(ctx, "onesweep32", bytes, [&] {
                 if (tile_items == 16)
                     k_onesweep<uint32_t, 16><<<tiles, kSortBlock, 0, ctx->stream>>>(
                         (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur],
-                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
                 else
                     k_onesweep<uint32_t, 8><<<tiles, kSortBlock, 0, ctx->stream>>>(
                         (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur],
-                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status);
+                        n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
             });
         }
         ATR_TRY(s);
@@ -1059,27 +1098,27 @@ atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_
         return launch(ctx, "build_refit63", bytes, [&] {
             k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, verts, t->parent, t->left,
                                                    t->right, t->data, t->bbmin, t->bbmax, t->area,
-                                                   t->span, slots);
+                                                   t->span, slots, ctx->d_status);
         });
     }
     SingleKeys<uint32_t> kv{(const uint32_t*)keys, (int)n - 1};
     return launch(ctx, "build_refit30", bytes, [&] {
         k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, verts, t->parent, t->left,
                                                t->right, t->data, t->bbmin, t->bbmax, t->area,
-                                               t->span, slots);
+                                               t->span, slots, ctx->d_status);
     });
 }
 
 atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32_t* idx,
                             const float4* verts, atr_tree* t, const uint32_t* d_seg,
-                            uint64_t* slots)
+                            uint32_t meshes, uint64_t* slots)
 {
     const uint32_t n = t->n;
-    ForestKeys kv{keys, (int)n - 1, d_seg};
+    ForestKeys kv{keys, (int)n - 1, d_seg, meshes};
     return launch(ctx, "build_refit_forest", (double)n * 176.0, [&] {
         k_build<<<div_up(n, 256), 256, 0, ctx->stream>>>(
             kv, idx, n, (int)t->first_leaf, verts, t->parent, t->left, t->right, t->data,
-            t->bbmin, t->bbmax, t->area, t->span, slots);
+            t->bbmin, t->bbmax, t->area, t->span, slots, ctx->d_status);
     });
 }
 

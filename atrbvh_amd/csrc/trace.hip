@@ -403,11 +403,24 @@ __device__ __forceinline__ void jenkins_mix(uint32_t& a, uint32_t& b, uint32_t& 
     c -= a; c -= b; c ^= (b >> 15);
 }
 
+// normalize() of the reference's vector library (framework/base/Math.hpp:
+// 139-141): v * rcp(sqrt((x*x + y*y) + z*z)) with rcp(0) = 0, so a zero
+// vector stays zero.
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z)
+{
+    const float l = sqrtf((x * x + y * y) + z * z);
+    const float r = l != 0.0f ? 1.0f / l : 0.0f;
+    x *= r;
+    y *= r;
+    z *= r;
+}
+
 // One cosine-weighted ray per primary ray (rayGenAOKernel with numSamples = 1,
 // gpu-ray-traversal/src/rt/ray/RayGenKernels.cu:114-221): origin backed off by
-// 1e-4 along the primary ray, geometric normal flipped towards the ray, frame
-// rotated by a hashed angle, Halton (2,3) sample 1 warped to the hemisphere.
-// A primary miss gives a degenerate ray (tmax = -1).
+// 1e-4 along the primary ray, the hit triangle's normal (Scene.cpp:74:
+// normalize(cross(v1 - v0, v2 - v0))) flipped towards the ray, frame rotated
+// by a hashed angle, Halton (2, 3) sample 1 warped to the hemisphere.  A
+// primary miss gives a degenerate ray (tmax = -1).
 __global__ __launch_bounds__(256) void k_rays_diffuse(const float4* __restrict__ v,
                                                       const float4* __restrict__ prim,
                                                       const int4* __restrict__ res, uint32_t R,
@@ -430,18 +443,9 @@ __global__ __launch_bounds__(256) void k_rays_diffuse(const float4* __restrict__
         nx = e1y * e2z - e1z * e2y;
         ny = e1z * e2x - e1x * e2z;
         nz = e1x * e2y - e1y * e2x;
-        const float len = sqrtf(nx * nx + ny * ny + nz * nz);
-        if (len > 0.0f) {
-            nx /= len;
-            ny /= len;
-            nz /= len;
-        } else {
-            nx = 1.0f;
-            ny = 0.0f;
-            nz = 0.0f;
-        }
+        normalize3(nx, ny, nz);
     }
-    if (nx * r1.x + ny * r1.y + nz * r1.z > 0.0f) {
+    if ((nx * r1.x + ny * r1.y) + nz * r1.z > 0.0f) {
         nx = -nx;
         ny = -ny;
         nz = -nz;
@@ -458,31 +462,26 @@ __global__ __launch_bounds__(256) void k_rays_diffuse(const float4* __restrict__
         py = 0.0f;
         pz = nx;
     }
-    const float pl = sqrtf(px * px + py * py + pz * pz);
-    px /= pl;
-    py /= pl;
-    pz /= pl;
+    normalize3(px, py, pz);
     const float bx = ny * pz - nz * py, by = nz * px - nx * pz, bz = nx * py - ny * px;
     uint32_t ha = seed + i, hb = 0x9e3779b9u, hc = 0x9e3779b9u;
     jenkins_mix(ha, hb, hc);
     jenkins_mix(ha, hb, hc);
-    const float angle = 2.0f * 3.14159265358979f * (float)hc * 2.3283064365386963e-10f;
+    const float kPi = 3.14159265358979323846f;
+    const float angle = 2.0f * kPi * (float)hc * 2.3283064365386963e-10f;
     const float ca = cosf(angle), sa = sinf(angle);
     const float t0x = px * ca + bx * sa, t0y = py * ca + by * sa, t0z = pz * ca + bz * sa;
-    const float t1x = -px * sa + bx * ca, t1y = -py * sa + by * ca, t1z = -pz * sa + bz * ca;
+    const float t1x = px * -sa + bx * ca, t1y = py * -sa + by * ca, t1z = pz * -sa + bz * ca;
     // Halton sample index 1: base 2 -> 0.5, base 3 -> 1/3
     const float hx = 0.5f, hy = 1.0f / 3.0f;
-    const float phi = 2.0f * 3.14159265358979f * hy;
+    const float phi = 2.0f * kPi * hy;
     const float rr = sqrtf(hx);
     const float sx = rr * cosf(phi), sy = rr * sinf(phi);
     const float sz = sqrtf(1.0f - sx * sx - sy * sy);
-    float dx = sx * t0x + sy * t1x + sz * nx;
-    float dy = sx * t0y + sy * t1y + sz * ny;
-    float dz = sx * t0z + sy * t1z + sz * nz;
-    const float dl = sqrtf(dx * dx + dy * dy + dz * dz);
-    dx /= dl;
-    dy /= dl;
-    dz /= dl;
+    float dx = (sx * t0x + sy * t1x) + sz * nx;
+    float dy = (sx * t0y + sy * t1y) + sz * ny;
+    float dz = (sx * t0z + sy * t1z) + sz * nz;
+    normalize3(dx, dy, dz);
     rays[2ull * i] = make_float4(ox, oy, oz, 0.0f);
     rays[2ull * i + 1] = make_float4(dx, dy, dz, tri >= 0 ? tmax : -1.0f);
 }

@@ -71,6 +71,31 @@ def traffic_from_profiles(kernel: str):
         return None
 
 
+def host_info() -> dict:
+    """What the CPU numbers were measured on: the host's logical CPU count, its
+    CPU model name and the compiler and flags of the CPU code (oracle/Makefile)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cc = "gcc"
+    try:
+        import subprocess
+
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True,
+                            timeout=10).stdout.splitlines()[0]
+    except Exception:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model,
+            "compiler": f"{cc}; C: -O2 -ffp-contract=off -fno-fast-math -std=c11; "
+                        "C++ (SBVH): -O2 -ffp-contract=off -fno-fast-math -std=c++17 -pthread"}
+
+
 def cpu_baseline(sample_tris: int, min_seconds: float = 10.0):
     """The oracle (serial C restatement of the reference build path) on the same
     workload, on one host core, repeated until at least min_seconds of work."""
@@ -87,7 +112,7 @@ def cpu_baseline(sample_tris: int, min_seconds: float = 10.0):
         runs.append(time.perf_counter() - t0)
     dt = float(np.median(runs))
     return {"value": round(sample_tris / dt / 1e6, 4), "unit": "Mtris/s", "cores": 1,
-            "kind": "port",
+            "kind": "port", **host_info(),
             "sample": f"oracle LBVH63 + ATRBVH(9,2) + collapse of the {sample_tris:,}-triangle "
                       f"GMM scene, single thread, median of {len(runs)} runs "
                       f"({sum(runs):.1f} s total)"}
@@ -102,7 +127,8 @@ def cpu_paths(v: np.ndarray, rays: np.ndarray | None, gpu_hits: np.ndarray | Non
     import oracle
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    out = {"threads": threads}
+    out = {"threads": threads, "threads_note": "host CPUs, capped at 16 (the box's CPU share)",
+           **host_info()}
     sample = 262144
     t0 = time.perf_counter()
     one = oracle.Sbvh(v[: 3 * sample], threads=1)

@@ -155,7 +155,17 @@ atr_status atr_tree_sah(atr_tree* tree, float ci, float ct, float* out);
 atr_status atr_tree_num_triangles(const atr_tree* tree, uint32_t* n);
 atr_status atr_tree_download(atr_tree* tree, atr_tree_view* host);
 atr_status atr_tree_device_view(atr_tree* tree, atr_tree_view* dev);
+/* Upload (HostToDeviceTree, BVHTreeInstanceManager.cpp:56-66) after the
+ * checks of atr_tree_view_check. */
 atr_status atr_tree_upload(atr_tree* tree, const atr_tree_view* host);
+/* Host-only validation of a tree view (no GPU needed): root 0; internal nodes
+ * 0..n-2 with data -1; leaves n-1..2n-2 with left = right = -1 and data a
+ * permutation of 0..n-1; every other node the child of exactly one internal
+ * node; parent[] consistent with the links and -1 at the root; every node
+ * reachable from the root.  The reference trusts its callers here
+ * (BVHTreeInstanceManager.cpp:56-66); every kernel of this library walks the
+ * links, so a tree that fails is rejected. */
+atr_status atr_tree_view_check(const atr_tree_view* host);
 atr_status atr_tree_destroy(atr_tree* tree);
 /* Tree from a host view (HostToDeviceTree, BVHTreeInstanceManager.cpp:56-66). */
 atr_status atr_tree_create_from_host(atr_context* ctx, const atr_tree_view* host,

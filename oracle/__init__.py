@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ATRBVH_ORACLE_LIB selects another build of the same sources (the mutation
+# tests load builds with a planted mistake through it).
+_LIB_PATH = os.environ.get("ATRBVH_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _REF_SCHED = os.path.join(_HERE, "_ref", "trbvh_sched_dump")
 
 _lib = None
@@ -263,6 +265,18 @@ def atrbvh(t: Tree, treelet=9, iterations=2, ci=1.2, ct=1.0) -> Tree:
     return t
 
 
+def treelet_pass(t: Tree, kind: str, treelet: int, gamma: int, ci=1.2, ct=1.0) -> int:
+    """One bottom-up treelet pass with threshold gamma (kind "trbvh" or
+    "atrbvh"), in place; returns the number of treelets processed."""
+    s = t.struct()
+    k = lib().orc_treelet_pass(ctypes.byref(s), 0 if kind == "trbvh" else 1, treelet, gamma,
+                               ctypes.c_float(ci), ctypes.c_float(ct))
+    if k < 0:
+        raise RuntimeError("oracle treelet pass failed")
+    t.pull(s)
+    return int(k)
+
+
 def collapse(t: Tree, ci=1.2, ct=1.0):
     c = Collapsed(t.n)
     cs = c.struct()
@@ -329,6 +343,18 @@ class Sbvh:
             self.close()
         except Exception:
             pass
+
+
+def rays_diffuse(v4, prim, res, seed, tmax=np.inf):
+    """rayGenAOKernel with one sample per ray (RayGenKernels.cu:114-221)."""
+    v4 = _f32(v4)
+    prim = _f32(prim)
+    res = np.ascontiguousarray(res, np.int32)
+    r = prim.shape[0]
+    out = np.zeros((r, 8), np.float32)
+    lib().orc_rays_diffuse(_p(v4, _f32p), _p(prim, _f32p), _p(res, _i32p), ctypes.c_uint32(r),
+                           ctypes.c_uint32(seed), ctypes.c_float(tmax), _p(out, _f32p))
+    return out
 
 
 def trace_brute(v4, rays):

@@ -6,6 +6,12 @@
  */
 #include "oracle.h"
 
+/* Test-only: ORC_MUTATION = k plants mistake k (see the #if blocks) so the
+ * composition tests can show they catch it; 0 in every normal build. */
+#ifndef ORC_MUTATION
+#define ORC_MUTATION 0
+#endif
+
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -586,11 +592,18 @@ static int form_treelet(orc_tree* t, int root, int n, int32_t* internal, int32_t
         int m = 2 + iter;
         float cand[32];
         for (int l = 0; l < m; ++l) cand[l] = is_leaf(leaves[l], N) ? -FLT_MAX : areas[l];
+#if ORC_MUTATION == 2 /* planted: the smallest internal candidate is expanded */
+        for (int l = 0; l < m; ++l)
+            if (!is_leaf(leaves[l], N)) cand[l] = FLT_MAX - areas[l];
+#endif
         int chosen = emulate_find_leaf_to_expand(m, cand);
         if (chosen < 0 || chosen >= m || is_leaf(leaves[chosen], N)) return -1;
         int replace = leaves[chosen];
         int l = t->left[replace];
         int r = t->right[replace];
+#if ORC_MUTATION == 1 /* planted: the expanded node's children change places */
+        { int tmp = l; l = r; r = tmp; }
+#endif
         internal[iter + 1] = replace;
         leaves[chosen] = l;
         leaves[iter + 2] = r;
@@ -618,6 +631,11 @@ float orc_trbvh_dp(int n, const float* leaf_cost, const float* subset_area, int 
     int nsub = 1 << n;
     int full = nsub - 1;
     float T = (float)treelet_tris;
+#if ORC_MUTATION == 3 /* planted: each subset's collapse term uses its own size */
+#define ORC_T_OF(s) ((float)popc32((uint32_t)(s)))
+#else
+#define ORC_T_OF(s) T
+#endif
     for (int s = 0; s < nsub; ++s) {
         costs[s] = subset_area[s];
         masks[s] = 0;
@@ -634,11 +652,17 @@ float orc_trbvh_dp(int n, const float* leaf_cost, const float* subset_area, int 
             int delta = (s - 1) & s;
             int p = (-delta) & s;
             while (p != 0) {
+#if ORC_MUTATION == 6 /* planted: the first partition of each subset is skipped */
+                if (p == ((-delta) & s) && ((p - delta) & s) != 0) {
+                    p = (p - delta) & s;
+                    continue;
+                }
+#endif
                 trbvh_pick_cost(costs[p] + costs[p ^ s], p, &best, &bestp);
                 p = (p - delta) & s;
             }
             float a = costs[s];
-            costs[s] = ORC_MIN(ci * a + best, ct * a * T);
+            costs[s] = ORC_MIN(ci * a + best, ct * a * ORC_T_OF(s));
             masks[s] = (int8_t)bestp;
         }
     }
@@ -678,7 +702,7 @@ float orc_trbvh_dp(int n, const float* leaf_cost, const float* subset_area, int 
             lp[0] = lp[1];
         }
         float a = costs[s];
-        costs[s] = ORC_MIN(ci * a + lc[0], ct * a * T);
+        costs[s] = ORC_MIN(ci * a + lc[0], ct * a * ORC_T_OF(s));
         masks[s] = (int8_t)lp[0];
     }
 
@@ -718,9 +742,10 @@ float orc_trbvh_dp(int n, const float* leaf_cost, const float* subset_area, int 
             memcpy(lp, np, sizeof(lp));
         }
         float a = costs[full];
-        costs[full] = ORC_MIN(ci * a + lc[0], ct * a * T);
+        costs[full] = ORC_MIN(ci * a + lc[0], ct * a * ORC_T_OF(full));
         masks[full] = (int8_t)lp[0];
     }
+#undef ORC_T_OF
     return costs[full];
 }
 
@@ -783,6 +808,9 @@ static int trbvh_treelet(orc_tree* t, int root, int n, const int32_t* tris, floa
                         t->area[child] = box_area(mn, mx);
                         /* both children receive costs[partition] (TRBVH.cu:275,309) */
                         nsah[child] = costs[p];
+#if ORC_MUTATION == 7 /* planted: the right child gets its own cost (quirk removed) */
+                        if (side == 1) nsah[child] = costs[part];
+#endif
                         nextNode[nsp] = child;
                         nextMask[nsp] = part;
                         nsp++;
@@ -800,6 +828,9 @@ static int trbvh_treelet(orc_tree* t, int root, int n, const int32_t* tris, floa
             memcpy(stackMask, nextMask, sizeof(int32_t) * nsp);
             sp = nsp;
         }
+#if ORC_MUTATION == 8 /* planted: the rewritten root's SAH is updated (quirk removed) */
+        nsah[internal[0]] = costs[full];
+#endif
     }
     return 0;
 }
@@ -894,6 +925,9 @@ static void agglomerate(agg_state* S, int n, const int32_t* internal, float ci, 
         memcpy(S->mx[c], amx, sizeof(amx));
         float area = box_area(amn, amx);
         S->areas[L] = sah + ci * area;
+#if ORC_MUTATION == 9 /* planted: the merged node's SAH drops its own area term */
+        S->areas[L] = sah;
+#endif
         S->sah[c] = S->areas[L];
 
         int32_t lastIndex = S->node[L];
@@ -923,6 +957,9 @@ static void agglomerate(agg_state* S, int n, const int32_t* internal, float ci, 
             for (int tt = 0; tt < L; ++tt) {
                 if (tt == c) continue;
                 int dr = c > tt ? c : tt, dc = c < tt ? c : tt;
+#if ORC_MUTATION == 5 /* planted: distances to the merged cluster are not refreshed */
+                continue;
+#endif
                 D[TRM_INDEX(dr, dc)] = union_area(S->mn[dr], S->mx[dr], S->mn[dc], S->mx[dc]);
             }
         }
@@ -1002,6 +1039,7 @@ typedef struct {
     int32_t* tris;
     float* nsah;
     int err;
+    int treelets; /* treelet roots processed in this pass */
 } pass_ctx;
 
 static void pass_visit(void* p, int32_t node)
@@ -1018,34 +1056,63 @@ static void pass_visit(void* p, int32_t node)
     c->tris[node] = count;
     c->nsah[node] = c->ci * t->area[node] + c->nsah[l] + c->nsah[r];
     if (count >= c->gamma) {
+        c->treelets++;
         int e = c->kind == 0 ? trbvh_treelet(t, node, c->n, c->tris, c->nsah, c->ci, c->ct)
                              : atrbvh_treelet(t, node, c->n, c->nsah, c->ci);
         if (e) c->err = e;
     }
 }
 
-static int treelet_optimize(orc_tree* t, int kind, int n, int iterations, float ci, float ct)
+/* One bottom-up pass with treelet threshold gamma: every node whose subtree
+ * holds at least gamma triangles when the pass reaches it roots a treelet.
+ * Returns the number of treelets, or -1. */
+static int treelet_pass(orc_tree* t, int kind, int n, int gamma, float ci, float ct)
 {
     size_t nodes = 2 * (size_t)t->n - 1;
     pass_ctx c;
     c.t = t;
     c.kind = kind;
     c.n = n;
+    c.gamma = gamma;
     c.ci = ci;
     c.ct = ct;
     c.err = 0;
+    c.treelets = 0;
     c.tris = (int32_t*)malloc(sizeof(int32_t) * nodes);
     c.nsah = (float*)malloc(sizeof(float) * nodes);
-    if (!c.tris || !c.nsah) return -1;
-    int gamma = n;
-    for (int it = 0; it < iterations; ++it) {
-        c.gamma = gamma;
-        if (postorder(t, &c, pass_visit)) c.err = -1;
-        gamma *= 2;
+    if (!c.tris || !c.nsah) {
+        free(c.tris);
+        free(c.nsah);
+        return -1;
     }
+    if (postorder(t, &c, pass_visit)) c.err = -1;
     free(c.tris);
     free(c.nsah);
-    return c.err;
+    return c.err ? -1 : c.treelets;
+}
+
+int orc_treelet_pass(orc_tree* t, int kind, int treelet, int gamma, float ci, float ct)
+{
+    if (kind == 0 && treelet != 7) return -1;
+    if (kind == 1 && (treelet < 3 || treelet > 32)) return -1;
+    if (t->n < 2 || gamma < 1) return -1;
+    return treelet_pass(t, kind, treelet, gamma, ci, ct);
+}
+
+/* The optimisers' schedule: iteration it uses gamma = n * 2^it
+ * (TRBVHOptimizer.cpp:91,105; AgglomerativeTreeletOptimizer.cpp:79-96). */
+static int treelet_optimize(orc_tree* t, int kind, int n, int iterations, float ci, float ct)
+{
+    int gamma = n;
+    for (int it = 0; it < iterations; ++it) {
+        if (treelet_pass(t, kind, n, gamma, ci, ct) < 0) return -1;
+#if ORC_MUTATION == 4 /* planted: gamma grows linearly */
+        gamma += n;
+#else
+        gamma *= 2;
+#endif
+    }
+    return 0;
 }
 
 /* TRBVHOptimizer::Optimize (TRBVHOptimizer.cpp:21-128); treelet size 7 only
@@ -1375,5 +1442,121 @@ void orc_trace_brute(const float* v4, uint32_t n, const float* rays, uint32_t r,
         results[i * 4 + 1] = fbits(tmax);
         results[i * 4 + 2] = fbits(best_u);
         results[i * 4 + 3] = fbits(best_v);
+    }
+}
+
+/* --- ray generation ------------------------------------------------------ */
+/* jenkinsMix (gpu-ray-traversal/src/rt/ray/RayGenKernels.cu:34-45): Bob
+ * Jenkins' 96-bit mix. */
+static void orc_jenkins_mix(uint32_t* a, uint32_t* b, uint32_t* c)
+{
+    *a -= *b; *a -= *c; *a ^= (*c >> 13);
+    *b -= *c; *b -= *a; *b ^= (*a << 8);
+    *c -= *a; *c -= *b; *c ^= (*b >> 13);
+    *a -= *b; *a -= *c; *a ^= (*c >> 12);
+    *b -= *c; *b -= *a; *b ^= (*a << 16);
+    *c -= *a; *c -= *b; *c ^= (*b >> 5);
+    *a -= *b; *a -= *c; *a ^= (*c >> 3);
+    *b -= *c; *b -= *a; *b ^= (*a << 10);
+    *c -= *a; *c -= *b; *c ^= (*b >> 15);
+}
+
+/* normalize() of framework/base/Math.hpp:139-141: v * (1 * rcp(length)),
+ * length = sqrt((x*x + y*y) + z*z), rcp(0) = 0 (Math.hpp:112). */
+static void orc_normalize3(float* v)
+{
+    const float l = sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    const float r = l != 0.0f ? 1.0f / l : 0.0f;
+    v[0] *= r;
+    v[1] *= r;
+    v[2] *= r;
+}
+
+/* rayGenAOKernel (RayGenKernels.cu:114-221) with numSamples = 1, one output
+ * ray per input ray, normals as Scene.cpp:74 computes them
+ * (normalize(cross(v1 - v0, v2 - v0))).  cosf/sinf are the host C library's;
+ * everything else is the kernel's fp32 arithmetic in its order. */
+void orc_rays_diffuse(const float* v4, const float* prim, const int32_t* res, uint32_t r,
+                      uint32_t seed, float tmax, float* out)
+{
+    const float pi = 3.14159265358979323846f; /* FW_PI, Defs.hpp:121 */
+    for (uint32_t i = 0; i < r; ++i) {
+        const float* o = prim + 8ull * i;
+        const float* d = o + 4;
+        float ht;
+        memcpy(&ht, &res[4ull * i + 1], 4);
+        const float back = fmaxf(ht - 1.0e-4f, 0.0f);
+        const float org[3] = {o[0] + d[0] * back, o[1] + d[1] * back, o[2] + d[2] * back};
+        const int tri = res[4ull * i];
+        float n[3] = {1.0f, 0.0f, 0.0f};
+        if (tri != -1) {
+            const float* a = v4 + 12ull * tri;
+            const float* b = a + 4;
+            const float* c = a + 8;
+            const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            orc_normalize3(n);
+        }
+        if ((n[0] * d[0] + n[1] * d[1]) + n[2] * d[2] > 0.0f) {
+            n[0] = -n[0];
+            n[1] = -n[1];
+            n[2] = -n[2];
+        }
+        const float na[3] = {fabsf(n[0]), fabsf(n[1]), fabsf(n[2])};
+        const float nm = fmaxf(fmaxf(na[0], na[1]), na[2]);
+        float p[3] = {n[1], -n[0], 0.0f};
+        if (nm == na[2]) {
+            p[0] = 0.0f;
+            p[1] = n[2];
+            p[2] = -n[1];
+        } else if (nm == na[0]) {
+            p[0] = -n[2];
+            p[1] = 0.0f;
+            p[2] = n[0];
+        }
+        orc_normalize3(p);
+        const float bp[3] = {n[1] * p[2] - n[2] * p[1], n[2] * p[0] - n[0] * p[2],
+                             n[0] * p[1] - n[1] * p[0]};
+        uint32_t ha = seed + i, hb = 0x9e3779b9u, hc = 0x9e3779b9u;
+        orc_jenkins_mix(&ha, &hb, &hc);
+        orc_jenkins_mix(&ha, &hb, &hc);
+        const float angle = 2.0f * pi * (float)hc * exp2f(-32.0f);
+        const float ca = cosf(angle), sa = sinf(angle);
+        float t0[3], t1[3];
+        for (int k = 0; k < 3; ++k) {
+            t0[k] = p[k] * ca + bp[k] * sa;
+            t1[k] = p[k] * -sa + bp[k] * ca;
+        }
+        /* sample 0: Halton base 2 of index 1 and base 3 of index 1 */
+        float x = 0.0f, xadd = 1.0f;
+        for (unsigned int h2 = 1; h2 != 0; h2 >>= 1) {
+            xadd *= 0.5f;
+            if (h2 & 1) x += xadd;
+        }
+        float y = 0.0f, yadd = 1.0f;
+        for (int h3 = 1; h3 != 0; h3 /= 3) {
+            yadd *= 1.0f / 3.0f;
+            y += (float)(h3 % 3) * yadd;
+        }
+        const float phi = 2.0f * pi * y;
+        const float rr = sqrtf(x);
+        x = rr * cosf(phi);
+        y = rr * sinf(phi);
+        const float z = sqrtf(1.0f - x * x - y * y);
+        float dir[3];
+        for (int k = 0; k < 3; ++k) dir[k] = (x * t0[k] + y * t1[k]) + z * n[k];
+        orc_normalize3(dir);
+        float* q = out + 8ull * i;
+        q[0] = org[0];
+        q[1] = org[1];
+        q[2] = org[2];
+        q[3] = 0.0f;
+        q[4] = dir[0];
+        q[5] = dir[1];
+        q[6] = dir[2];
+        q[7] = tri == -1 ? -1.0f : tmax;
     }
 }

@@ -108,6 +108,10 @@ int orc_atrbvh_schedule(int treelet, int warp, int32_t* out, int cap);
 int orc_trbvh_optimize(orc_tree* t, int treelet, int iterations, float ci, float ct);
 int orc_atrbvh_optimize(orc_tree* t, int treelet, int iterations, float ci, float ct);
 
+/* One bottom-up pass (kind 0 = TRBVH, 1 = ATRBVH) with an explicit treelet
+ * threshold gamma; returns the number of treelets it processed, or -1. */
+int orc_treelet_pass(orc_tree* t, int kind, int treelet, int gamma, float ci, float ct);
+
 /* Single-treelet primitives exposed for the closed-form tests. */
 float orc_trbvh_dp(int n, const float* leaf_cost, const float* subset_area, int treelet_tris,
                    float ci, float ct, float* costs_out, int32_t* masks_out);
@@ -133,6 +137,12 @@ void orc_trace_brute(const float* v4, uint32_t n, const float* rays, uint32_t r,
 void orc_ray_inverse(const float* d, float* inv);
 int orc_ray_triangle(const float* v0, const float* v1, const float* v2, const float* o,
                      const float* d, float tmin, float tmax, float* t, float* u, float* v);
+
+/* --- ray generation ----------------------------------------------------- */
+/* One diffuse ray per primary ray (rayGenAOKernel, RayGenKernels.cu:114-221,
+ * numSamples = 1); prim float4[2r], res int32[4r] as orc_trace writes them. */
+void orc_rays_diffuse(const float* v4, const float* prim, const int32_t* res, uint32_t r,
+                      uint32_t seed, float tmax, float* out);
 
 /* CPU baseline of the reference's CPU path (oracle/sbvh.cpp): SBVH build
  * (SplitBVHBuilder.cpp:52-476, "GPU" platform, leaves 1..8, alpha 1e-5) on

@@ -6,9 +6,9 @@
   4, 8, 32, 64.  Only the printed integers are stored.
 - c1_summary.json: the oracle's results on the 65,536-triangle soup workload
   (seed 1): scene box, SAH after LBVH / ATRBVH(9,2) / TRBVH(7,3) / collapse,
-  and SHA-256 digests of every tree array, as a regression pin.
-- tiny_trees.json: complete LBVH / ATRBVH / collapsed trees of a 64-triangle
-  soup (seed 5) from the oracle.
+  and SHA-256 digests of every tree array.  A regression guard against
+  unintended changes of the oracle, not a pin: it holds the oracle's own
+  output.
 
 Run from the repository root: python tests/golden/make_fixtures.py
 """
@@ -73,22 +73,6 @@ def main():
     with open(os.path.join(HERE, "c1_summary.json"), "w") as f:
         json.dump(summary, f, indent=1)
 
-    v = ab.gen_soup(5, 64)
-    mn, mx = oracle.scene_bbox(v)
-    lb = oracle.lbvh(v, mn, mx, 30)
-    at = lb.copy()
-    oracle.atrbvh(at, 9, 2)
-    c, csah = oracle.collapse(at)
-
-    def dump(t):
-        return {"parent": t.parent.tolist(), "left": t.left.tolist(), "right": t.right.tolist(),
-                "data": t.data.tolist(), "area": [f32(x) for x in t.area]}
-
-    tiny = {"vertices": [f32(x) for x in v.reshape(-1)], "lbvh": dump(lb), "atrbvh": dump(at),
-            "collapsed": {"data": c.data.tolist(), "count": c.count.tolist(),
-                          "tri_index": c.tri_index[: c.used].tolist(), "sah": f32(csah)}}
-    with open(os.path.join(HERE, "tiny_trees.json"), "w") as f:
-        json.dump(tiny, f)
     print("fixtures written")
 
 

@@ -136,3 +136,72 @@ def test_batch_generators():
     for m in range(8):
         vm = v[3 * int(small[m]): 3 * int(small[m + 1])]
         assert np.array_equal(vm, ab.gen_soup(3000 + m, int(small[m + 1] - small[m])))
+
+
+def _view(t: dict):
+    keep = {k: np.ascontiguousarray(t[k], np.int32) for k in ("parent", "left", "right", "data")}
+    v = ab.TreeView(int(t["n"]), 0, ab._ptr(keep["parent"], ab._i32p),
+                    ab._ptr(keep["left"], ab._i32p), ab._ptr(keep["right"], ab._i32p),
+                    ab._ptr(keep["data"], ab._i32p), None, None, None)
+    return v, keep
+
+
+def _oracle_tree(n=64, seed=5):
+    v = ab.gen_soup(seed, n)
+    mn, mx = oracle.scene_bbox(v)
+    t = oracle.lbvh(v, mn, mx, 30)
+    oracle.atrbvh(t, 9, 1)
+    return {"n": n, "parent": t.parent.copy(), "left": t.left.copy(), "right": t.right.copy(),
+            "data": t.data.copy()}
+
+
+def _check(t: dict) -> int:
+    v, keep = _view(t)
+    return ab.lib().atr_tree_view_check(ctypes.byref(v))
+
+
+def test_tree_view_check_accepts_valid_trees():
+    assert _check(_oracle_tree()) == 0
+    assert _check(_oracle_tree(2, 9)) == 0
+
+
+def test_tree_view_check_rejects_bad_parent_array():
+    t = _oracle_tree()
+    t["parent"][t["left"][0]] = 3  # a child whose parent entry names another node
+    assert _check(t) == 1
+    assert b"parent" in ab.lib().atr_last_error()
+    t = _oracle_tree()
+    t["parent"][0] = 0
+    assert _check(t) == 1
+
+
+def test_tree_view_check_rejects_cycles_and_bad_leaves():
+    n = 64
+    t = _oracle_tree(n)
+    # Detach the root's left subtree and make a two-node cycle of internal
+    # nodes: every non-root node is still a child exactly once and every
+    # parent entry matches, but the cycle is unreachable from the root.
+    a = int(t["left"][0])
+    assert a < n - 1
+    b = int(t["left"][a])
+    if b >= n - 1:
+        b = int(t["right"][a])
+    assert b < n - 1, "need two internal nodes"
+    # root.left <- b.left and b.left <- a: a and b are each other's parent
+    bl = int(t["left"][b])
+    t["left"][0] = bl
+    t["parent"][bl] = 0
+    t["left"][b] = a
+    t["parent"][a] = b
+    assert _check(t) == 1
+    assert b"unreachable" in ab.lib().atr_last_error()
+    t = _oracle_tree(n)
+    leaf = n - 1 + 3
+    t["left"][leaf] = 1
+    assert _check(t) == 1
+    t = _oracle_tree(n)
+    t["data"][n - 1] = t["data"][n]  # duplicate triangle
+    assert _check(t) == 1
+    t = _oracle_tree(n)
+    t["data"][0] = 7  # internal node with data
+    assert _check(t) == 1

@@ -231,3 +231,42 @@ def test_autotuner_launch_shapes_keep_trees(ctx):
     tr = tune_treelet(ctx, b, sah_tol=0.05, reps=1)
     assert tr["best"]["mean_sah"] <= tr["sah_limit"]
     b.close()
+
+
+def test_batch_rebuild_keeps_given_boxes(ctx):
+    """A batch built on caller boxes rebuilds its meshes in those same frames:
+    after a step that moves vertices, a selective rebuild gives exactly the
+    trees of a fresh batch of the moved vertices on the same boxes (computed
+    boxes would change the Morton quantisation and the trees)."""
+    v, off = _uneven_batch(9, 6, 200, 5000, extra_small=False)
+    boxes = []
+    for m in range(6):
+        vm = v[3 * int(off[m]): 3 * int(off[m + 1])]
+        mn, mx = oracle.scene_bbox(vm)
+        boxes.append(np.concatenate([mn - 0.25, mx + 0.25]))
+    boxes = np.array(boxes, np.float32)
+    b = ab.Batch(ctx, v, off, boxes)
+    b.build(9, 2, graph=False)
+    n = int(off[-1])
+    g = np.random.default_rng(4).normal(size=(3 * n, 3)).astype(np.float32)
+    d_g = ctx.to_device(g)
+    b.step(d_g, 0.01)
+    moved = v.copy()
+    moved[:, :3] -= np.float32(0.01) * g
+    b.rebuild([2, 4])
+    fresh = ab.Batch(ctx, moved, off, boxes)
+    fresh.build(9, 2, graph=False)
+    for m in (2, 4):
+        t = b.extract(m)
+        f = fresh.extract(m)
+        assert_tree_equal(t.download(), f.download(), f"given-box rebuild {m}")
+        t.close()
+        f.close()
+    vm = moved[3 * int(off[2]): 3 * int(off[3])]
+    ref = oracle.lbvh(vm, boxes[2, :3], boxes[2, 3:], 30)
+    oracle.atrbvh(ref, 9, 2)
+    t = b.extract(2)
+    assert_tree_equal(t.download(), tree_arrays(ref), "given-box rebuild vs oracle")
+    t.close()
+    fresh.close()
+    b.close()

@@ -254,12 +254,17 @@ def test_trace_primary_c1(ctx):
     # brute force on a subset pins the oracle traversal itself
     sub = slice(0, r, 97)
     compare_hits(rr[sub], oracle.trace_brute(v, rays[sub]))
-    # diffuse secondary rays from the primary hits
+    # diffuse secondary rays from the primary hits, against the oracle's
+    # restatement of rayGenAOKernel (directions within 2e-6: cosf/sinf)
     d_sec = ctx.buffer(r * 32)
     ctx.rays_diffuse(s, d_rays, d_res, r, 12345, np.inf, d_sec)
     d_res2 = ctx.buffer(r * 16)
     c.trace(d_sec, r, d_res2)
     sec = d_sec.download(np.float32, (r, 8))
+    want = oracle.rays_diffuse(v, rays, dev, 12345)
+    assert np.array_equal(bits(sec[:, 0:4]), bits(want[:, 0:4]))
+    assert np.array_equal(bits(sec[:, 7]), bits(want[:, 7]))
+    assert np.max(np.abs(sec[:, 4:7] - want[:, 4:7])) <= 2e-6
     compare_hits(d_res2.download(np.int32, (r, 4)), oracle.trace(rc, v, sec))
     c.close()
     t.close()

@@ -617,3 +617,46 @@ def test_sbvh_trace_equals_brute_force_and_threads_agree():
     assert np.array_equal(got1, got8)
     compare_hits(got1, ref)
     assert (ref[:, 0] >= 0).mean() > 0.15
+
+
+def test_rays_diffuse_restatement_invariants():
+    """orc_rays_diffuse against what the mathematics fixes: sample 1 of the
+    Halton (2, 3) sequence is (1/2, 1/3), so the cosine warp puts every ray at
+    r = sqrt(1/2) from the pole: its component along the (ray-facing) normal
+    is sqrt(1/2) and the in-plane part has length sqrt(1/2) at polar angle
+    2*pi/3 relative to the hashed frame; directions are unit vectors; the
+    origin is backed off 1e-4 along the primary ray; misses give tmax = -1.
+    A dropped normal flip, a swapped Halton base or a missing back-off fails."""
+    rng = np.random.default_rng(3)
+    n = 64
+    v = np.zeros((3 * n, 4), np.float32)
+    v[:, :3] = rng.uniform(-1, 1, (3 * n, 3)).astype(np.float32)
+    r = 4096
+    prim = np.zeros((r, 8), np.float32)
+    prim[:, 0:3] = rng.uniform(-2, 2, (r, 3))
+    d = rng.normal(size=(r, 3))
+    prim[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    prim[:, 7] = np.inf
+    res = np.zeros((r, 4), np.int32)
+    res[:, 0] = rng.integers(-1, n, r)
+    tt = rng.uniform(0.0, 3.0, r).astype(np.float32)
+    tt[:5] = [0.0, 5e-5, 1e-4, 2e-4, 1.0]
+    res[:, 1] = tt.view(np.int32)
+    out = oracle.rays_diffuse(v, prim, res, seed=77)
+    hit = res[:, 0] >= 0
+    assert np.all(out[~hit, 7] == -1.0) and np.all(out[hit, 7] == np.inf)
+    back = np.maximum(tt.astype(np.float64) - 1e-4, 0.0)
+    assert np.allclose(out[:, 0:3], prim[:, 0:3] + prim[:, 4:7] * back[:, None], atol=1e-6)
+    dirs = out[:, 4:7].astype(np.float64)
+    assert np.allclose(np.linalg.norm(dirs, axis=1), 1.0, atol=1e-6)
+    tri = v[:, :3].reshape(n, 3, 3).astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    nn = np.where(hit[:, None], nrm[np.maximum(res[:, 0], 0)], [1.0, 0.0, 0.0])
+    flip = np.sum(nn * prim[:, 4:7], axis=1) > 0
+    nn[flip] *= -1
+    cos_pole = np.sum(dirs * nn, axis=1)
+    assert np.allclose(cos_pole, np.sqrt(0.5), atol=2e-6)
+    # the hashed rotation angle makes the in-plane direction vary from ray to ray
+    inplane = dirs - cos_pole[:, None] * nn
+    assert np.std(inplane[:, 0]) > 0.2
