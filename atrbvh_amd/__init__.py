@@ -116,6 +116,7 @@ def lib():
         L.atr_morton_encode.argtypes = [_vp, _vp, ctypes.c_uint32, _f32p, _f32p, ctypes.c_int,
                                         _vp, _vp]
         L.atr_sort_pairs.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32]
+        L.atr_sort_pairs_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32]
         L.atr_refit.argtypes = [_vp, _vp]
         L.atr_gen_soup.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_float, _f32p]
         L.atr_gen_soup.restype = None

@@ -83,7 +83,7 @@ static hipEvent_t take_event(atr_context* ctx)
 
 LaunchScope::LaunchScope(atr_context* c, const char* name, double bytes) : ctx(c), rec(-1)
 {
-    if (!ctx->profiling || ctx->capturing) return;
+    if (!ctx->profiling || ctx->capturing || ctx->no_alloc) return;
     auto it = ctx->prof_index.find(name);
     if (it == ctx->prof_index.end()) {
         ctx->prof.emplace_back();
@@ -167,6 +167,12 @@ atr_status atr_context_create(int device, atr_context** out)
         return fail(ATR_ERR_DEVICE, "context setup failed");
     }
     for (int i = kPinnedSlots - 1; i >= 0; --i) ctx->pinned_free.push_back(i);
+    // The no-allocation entry points' scratch: a failed lookup only disables
+    // them (they report it), the rest of the library does not need it.
+    if (noalloc_setup(device, &ctx->noalloc_scratch) != ATR_OK) {
+        ctx->noalloc_error = g_last_error;
+        ctx->noalloc_scratch = nullptr;
+    }
     *out = ctx;
     return ATR_OK;
 }

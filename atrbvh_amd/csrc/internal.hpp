@@ -59,6 +59,9 @@ struct atr_context {
     // true while the stream is being captured into a HIP graph (no profiling
     // events are recorded then)
     bool capturing = false;
+    // true inside an entry point that must allocate nothing (no profiling
+    // events are created or recorded then either)
+    bool no_alloc = false;
     // device status word: bit 0 = a bounded wait gave up, bit 1 = a tree was
     // deeper than a traversal stack; cleared when atr_sync reports it
     uint32_t* d_status = nullptr;
@@ -67,6 +70,9 @@ struct atr_context {
     uint32_t spin_limit = 0;
     // compute units of the device (0 until first queried)
     int cu_count = 0;
+    // this device's copy of the no-allocation scratch array (noalloc.hip)
+    void* noalloc_scratch = nullptr;
+    std::string noalloc_error;  // why the lookup failed, if it did
     // pinned host staging for small readbacks, handed out in fixed slots
     void* h_pinned = nullptr;
     std::vector<int> pinned_free;
@@ -297,6 +303,10 @@ atr_status run_rays_pinhole(atr_context* ctx, const float* eye, const float* bas
 atr_status run_rays_diffuse(atr_context* ctx, const float4* verts, const float4* primary,
                             const int32_t* results, uint32_t r, uint32_t seed, float tmax,
                             float4* rays);
+
+// No-allocation entry points (noalloc.hip): the device address of this
+// device's scratch array, looked up once per device.
+atr_status noalloc_setup(int device, void** scratch);
 
 // Host schedules (schedule.cpp)
 int trbvh_schedule(int treelet, int warp, std::vector<int32_t>& out);

@@ -221,6 +221,15 @@ atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t
  * written. */
 atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t* d_values,
                           uint32_t n);
+/* atr_sort_pairs without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its scratch is a fixed 24 MiB device array per GPU, held until the process
+ * ends; at most 1,048,576 pairs (larger n => ATR_ERR_INVALID_ARGUMENT).  Calls
+ * that overlap on one GPU (two contexts or host threads on the same device)
+ * race on that array.  Profiling events are not recorded for it. */
+atr_status atr_sort_pairs_noalloc(atr_context* ctx, int key_bits, void* d_keys,
+                                  uint32_t* d_values, uint32_t n);
 /* Bottom-up refit of every box and area of a tree over the scene vertices
  * (calculateNodeBoundingBoxesKernel semantics, BVHRT-Core/src/LBVH.cu:465-570)
  * with the topology left unchanged. */

@@ -205,3 +205,8 @@ def test_tree_view_check_rejects_cycles_and_bad_leaves():
     t = _oracle_tree(n)
     t["data"][0] = 7  # internal node with data
     assert _check(t) == 1
+
+
+def test_no_alloc_sort_argument_validation_without_device():
+    L = ab.lib()
+    assert L.atr_sort_pairs_noalloc(None, 32, None, None, 0) == 1
