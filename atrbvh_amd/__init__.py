@@ -118,6 +118,15 @@ def lib():
         L.atr_sort_pairs.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32]
         L.atr_sort_pairs_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32]
         L.atr_refit.argtypes = [_vp, _vp]
+        L.atr_mesh_load_obj.argtypes = [ctypes.c_char_p, ctypes.POINTER(_vp)]
+        L.atr_mesh_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.POINTER(ctypes.c_uint32)]
+        L.atr_mesh_soup.argtypes = [_vp, _vp]
+        L.atr_mesh_destroy.argtypes = [_vp]
+        L.atr_scene_create_from_mesh.argtypes = [_vp, _vp, ctypes.POINTER(_vp)]
+        L.atr_collapsed_compact.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(CompactLayout)]
+        L.atr_trace_compact.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp, _vp, _vp, _vp]
         L.atr_gen_soup.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_float, _f32p]
         L.atr_gen_soup.restype = None
         L.atr_gen_gmm.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
@@ -195,6 +204,42 @@ def gen_soup_batch(seed_base: int, offsets: np.ndarray, sigma: float = 0.005) ->
     v = np.zeros((3 * int(off[-1]), 4), np.float32)
     lib().atr_gen_soup_batch(seed_base, off.size - 1, _ptr(off, _u32p), sigma, _ptr(v))
     return v
+
+
+class Mesh:
+    """A Wavefront OBJ scene read as the reference's importer reads it."""
+
+    def __init__(self, path: str):
+        h = _vp()
+        check(lib().atr_mesh_load_obj(os.fsencode(path), ctypes.byref(h)))
+        self.h = h.value
+
+    def info(self) -> dict:
+        n = ctypes.c_uint32()
+        p = ctypes.c_uint32()
+        check(lib().atr_mesh_info(self.h, ctypes.byref(n), ctypes.byref(p)))
+        return {"triangles": n.value, "positions": p.value}
+
+    def soup(self) -> np.ndarray:
+        v = np.zeros((3 * self.info()["triangles"], 4), np.float32)
+        if v.size:
+            check(lib().atr_mesh_soup(self.h, v.ctypes.data))
+        return v
+
+    def scene(self, ctx: "Context") -> "Scene":
+        """SceneConverter::ConvertToCudaSceneDeviceMemory."""
+        h = _vp()
+        check(lib().atr_scene_create_from_mesh(ctx.h, self.h, ctypes.byref(h)))
+        s = Scene.__new__(Scene)
+        s.ctx = ctx
+        s.n = self.info()["triangles"]
+        s.h = h.value
+        return s
+
+    def close(self):
+        if self.h:
+            lib().atr_mesh_destroy(self.h)
+            self.h = None
 
 
 def trbvh_schedule(treelet: int = 7, warp: int = 32) -> np.ndarray:
@@ -321,6 +366,13 @@ class Context:
         check(lib().atr_mesh_step(self.h, d_mesh.ptr, n_vertices,
                                   d_grad.ptr if d_grad is not None else None, lr, max_step,
                                   d_index.ptr, scene.h))
+
+    def trace_compact(self, n: int, d_rays: DeviceBuffer, d_results: DeviceBuffer,
+                      layout: "CompactLayout", any_hit: bool = False):
+        """The reference's `trace` kernel ABI on a Compact2 layout."""
+        check(lib().atr_trace_compact(self.h, n, 1 if any_hit else 0, d_rays.ptr, d_results.ptr,
+                                      layout.nodes, None, None, None, layout.tris, None, None,
+                                      layout.tri_indices))
 
     def rays_pinhole(self, eye, basis, tan_half: float, w: int, h: int, tmax: float,
                      d_rays: DeviceBuffer):
@@ -473,15 +525,39 @@ class Collapsed:
     def trace(self, d_rays: DeviceBuffer, n: int, d_results: DeviceBuffer, any_hit=False):
         check(lib().atr_trace(self.h, d_rays.ptr, n, 1 if any_hit else 0, d_results.ptr))
 
+    def compact(self, node_offset_size_div: int = 16) -> CompactLayout:
+        """CudaBVH::createCompact on the GPU (Compact2 by default)."""
+        out = CompactLayout()
+        check(lib().atr_collapsed_compact(self.h, node_offset_size_div, ctypes.byref(out)))
+        return out
+
     def close(self):
         if self.h:
             lib().atr_collapsed_destroy(self.h)
             self.h = None
 
 
+class CompactLayout(ctypes.Structure):
+    """atr_compact_layout: the reference tracer's layout on the device."""
+    _fields_ = [("nodes", _vp), ("num_nodes", ctypes.c_uint32), ("tris", _vp),
+                ("tri_indices", _vp), ("num_rows", ctypes.c_uint32),
+                ("node_offset_size_div", ctypes.c_int)]
+
+    def download(self, ctx: "Context"):
+        """(nodes int32[M, 16], Woop rows float32[R, 4], tri_indices int32[R])."""
+        nodes = np.empty((self.num_nodes, 16), np.int32)
+        woop = np.empty((self.num_rows, 4), np.float32)
+        tri = np.empty(self.num_rows, np.int32)
+        check(lib().atr_memcpy_d2h(ctx.h, nodes.ctypes.data, self.nodes, nodes.nbytes))
+        check(lib().atr_memcpy_d2h(ctx.h, woop.ctypes.data, self.tris, woop.nbytes))
+        check(lib().atr_memcpy_d2h(ctx.h, tri.ctypes.data, self.tri_indices, tri.nbytes))
+        return nodes, woop, tri
+
+
 class BatchConfig(ctypes.Structure):
     """atr_batch_config: launch shape of batched builds."""
-    _fields_ = [("encode_items", ctypes.c_int32), ("sort_items", ctypes.c_int32)]
+    _fields_ = [("encode_items", ctypes.c_int32), ("sort_items", ctypes.c_int32),
+                ("treelet_block", ctypes.c_int32)]
 
 
 class Batch:
@@ -513,14 +589,15 @@ class Batch:
         check(lib().atr_batch_sah(self.h, ci, ct, _ptr(out)))
         return out
 
-    def set_config(self, encode_items: int = 16, sort_items: int = 16):
-        cfg = BatchConfig(encode_items, sort_items)
+    def set_config(self, encode_items: int = 16, sort_items: int = 16, treelet_block: int = 128):
+        cfg = BatchConfig(encode_items, sort_items, treelet_block)
         check(lib().atr_batch_set_config(self.h, ctypes.byref(cfg)))
 
     def config(self) -> dict:
         cfg = BatchConfig()
         check(lib().atr_batch_get_config(self.h, ctypes.byref(cfg)))
-        return {"encode_items": cfg.encode_items, "sort_items": cfg.sort_items}
+        return {"encode_items": cfg.encode_items, "sort_items": cfg.sort_items,
+                "treelet_block": cfg.treelet_block}
 
     def trace(self, d_rays: DeviceBuffer, rays_per_mesh: int, d_hits: DeviceBuffer):
         check(lib().atr_batch_trace(self.h, d_rays.ptr, rays_per_mesh, d_hits.ptr))

@@ -2,9 +2,10 @@
 
 Two searches, both measured on the caller's own batch with graph replays:
 
-- tune_launch: the launch shape (triangles per encode thread, keys per
-  radix-sort thread: 8 or 16 each).  Every shape computes the same trees; the
-  tuner checks that the per-mesh SAH of every candidate equals the first
+- tune_launch: the launch shape (triangles per encode thread and keys per
+  radix-sort thread: 8 or 16 each; threads per workgroup of the treelet
+  passes: 64, 128 or 256).  Every shape computes the same trees; the tuner
+  checks that the per-mesh SAH of every candidate equals the first
   candidate's bit for bit and keeps the fastest shape on the batch.
 - tune_treelet: treelet size and restructuring iterations, which do change
   the trees: the fastest candidate whose mean per-mesh SAH stays within
@@ -16,7 +17,10 @@ import time
 
 import numpy as np
 
-LAUNCH_SHAPES = ((16, 16), (16, 8), (8, 16), (8, 8))  # (encode_items, sort_items)
+import itertools
+
+# (encode_items, sort_items, treelet_block)
+LAUNCH_SHAPES = tuple(itertools.product((16, 8), (16, 8), (128, 64, 256)))
 TREELET_CANDIDATES = ((9, 2), (7, 2), (9, 1), (11, 1), (11, 2), (13, 1))
 
 
@@ -30,24 +34,31 @@ def _time_builds(ctx, batch, treelet: int, iterations: int, reps: int) -> float:
     return (time.perf_counter() - t0) * 1000.0 / reps
 
 
-def tune_launch(ctx, batch, treelet: int = 9, iterations: int = 2, reps: int = 3) -> dict:
+def tune_launch(ctx, batch, treelet: int = 9, iterations: int = 2, reps: int = 3,
+                shapes=LAUNCH_SHAPES) -> dict:
     """Times every launch shape, checks identical per-mesh SAH, keeps the fastest
-    shape set on the batch.  Returns {"best": {...}, "table": [...]}."""
+    shape set on the batch.  Returns {"best": {...}, "table": [...],
+    "gain_vs_default": default time / best time} (the default shape is the
+    first one)."""
     table = []
     ref_sah = None
-    for enc, srt in LAUNCH_SHAPES:
-        batch.set_config(encode_items=enc, sort_items=srt)
+    for enc, srt, blk in shapes:
+        batch.set_config(encode_items=enc, sort_items=srt, treelet_block=blk)
         ms = _time_builds(ctx, batch, treelet, iterations, reps)
         sah = batch.sah()
         if ref_sah is None:
             ref_sah = sah
         same = bool(np.array_equal(sah.view(np.uint32), ref_sah.view(np.uint32)))
         if not same:
-            raise RuntimeError(f"launch shape {enc}/{srt} changed the trees")
-        table.append({"encode_items": enc, "sort_items": srt, "ms": round(ms, 3)})
+            raise RuntimeError(f"launch shape {enc}/{srt}/{blk} changed the trees")
+        table.append({"encode_items": enc, "sort_items": srt, "treelet_block": blk,
+                      "ms": round(ms, 3)})
     best = min(table, key=lambda r: r["ms"])
-    batch.set_config(encode_items=best["encode_items"], sort_items=best["sort_items"])
-    return {"best": best, "table": table}
+    batch.set_config(encode_items=best["encode_items"], sort_items=best["sort_items"],
+                     treelet_block=best["treelet_block"])
+    default = table[0]
+    return {"best": best, "table": table,
+            "gain_vs_default": round(default["ms"] / best["ms"], 4)}
 
 
 def tune_treelet(ctx, batch, sah_tol: float = 0.01, reps: int = 3,

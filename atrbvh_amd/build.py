@@ -40,7 +40,7 @@ DEVICE = [
 ]
 
 SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "batch.hip", "adjoint.hip",
-           "noalloc.hip", "api.cpp",
+           "noalloc.hip", "compact.hip", "api.cpp", "obj.cpp",
            "scenes.cpp", "schedule.cpp"]
 
 # Per-file flags.  treelet.hip only ever handles finite boxes, areas and SAH
@@ -78,7 +78,22 @@ def build(jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_plain_loop()
     return LIB
+
+
+PLAIN_LOOP = os.path.join(HERE, "plain_loop")
+
+
+def build_plain_loop() -> str:
+    """The C++ per-mesh loop the benchmark times beside the batched build
+    (host code only, against the library's C ABI)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", os.path.join(HERE, "plain_loop.cpp"),
+           "-L" + HERE, "-latrbvh", "-Wl,-rpath,$ORIGIN", "-o", PLAIN_LOOP]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"plain_loop build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return PLAIN_LOOP
 
 
 if __name__ == "__main__":

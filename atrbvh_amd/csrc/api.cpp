@@ -207,6 +207,7 @@ atr_status atr_sync(atr_context* ctx)
         return fail(ATR_ERR_INTERNAL,
                     "a bounded device wait gave up (the sort's results were discarded and "
                     "trees built since the last synchronisation use input order)");
+    if (st & 4u) return fail(ATR_ERR_INTERNAL, "inconsistent traversal layout counts");
     return fail(ATR_ERR_UNSUPPORTED, "a tree was deeper than the traversal stack");
 }
 
@@ -779,6 +780,9 @@ atr_status atr_collapsed_destroy(atr_collapsed* c)
     dfree(ctx, c->tnodes);
     dfree(ctx, c->ttris);
     dfree(ctx, c->ttri_id);
+    dfree(ctx, c->cnodes);
+    dfree(ctx, c->cwoop);
+    dfree(ctx, c->crows_index);
     dfree(ctx, c->d_summary);
     // The event and pinned slot go back to their pools without waiting: a
     // later user records / copies after this tree's pending work in stream
@@ -806,6 +810,7 @@ atr_status atr_collapse(atr_tree* t, float ci, float ct, atr_collapsed** out, fl
     atr_collapsed* c = nullptr;
     ATR_TRY(collapsed_alloc(ctx, t->n, &c));
     c->has_layout = verts != nullptr;
+    c->verts = verts;
     atr_status s = run_collapse(ctx, t, c, ci, ct, verts);
     if (s == ATR_OK) {
         CollapseSummary* dst = (CollapseSummary*)ctx->h_pinned + c->pinned_slot;
@@ -871,6 +876,42 @@ atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, i
     ATR_TRY(collapsed_summary(c));
     return run_trace(c->ctx, c, (const float4*)d_rays, num_rays, any_hit, d_results,
                      c->h_summary.height);
+}
+
+atr_status atr_collapsed_compact(atr_collapsed* c, int node_offset_size_div,
+                                 atr_compact_layout* out)
+{
+    if (!c || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (node_offset_size_div != 1 && node_offset_size_div != 16)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "node_offset_size_div must be 1 or 16");
+    if (!c->verts)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "collapsed tree has no vertices: its tree was not bound to a scene");
+    ATR_TRY(run_compact(c->ctx, c, c->verts, node_offset_size_div));
+    out->nodes = (const float*)c->cnodes;
+    out->num_nodes = c->compact_nodes;
+    out->tris = (const float*)c->cwoop;
+    out->tri_indices = c->crows_index;
+    out->num_rows = c->compact_rows;
+    out->node_offset_size_div = c->compact_div;
+    return ATR_OK;
+}
+
+atr_status atr_trace_compact(atr_context* ctx, int num_rays, int any_hit, const float* rays,
+                             int32_t* results, const float* nodesA, const float* nodesB,
+                             const float* nodesC, const float* nodesD, const float* trisA,
+                             const float* trisB, const float* trisC, const int32_t* triIndices)
+{
+    (void)nodesB;
+    (void)nodesC;
+    (void)nodesD;
+    (void)trisB;
+    (void)trisC;
+    if (!ctx || (num_rays > 0 && (!rays || !results || !nodesA || !trisA || !triIndices)))
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (num_rays < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "num_rays must be >= 0");
+    return run_trace_compact(ctx, num_rays, any_hit, (const float4*)rays, (int2*)results,
+                             (const float4*)nodesA, (const float4*)trisA, triIndices);
 }
 
 atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float basis[9],
@@ -1134,7 +1175,7 @@ static atr_status batch_enqueue(atr_batch* b, int treelet, int iterations, float
     if (iterations > 0) ATR_TRY(run_make_records(ctx, b->forest, b->rec));
     for (int it = 0; it < iterations; ++it) {
         ATR_TRY(run_atrbvh_pass(ctx, b->forest, treelet, gamma, ci, ct, b->tris, b->nsah,
-                                b->counters, b->rec));
+                                b->counters, b->rec, b->treelet_block));
         gamma *= 2;
     }
     return ATR_OK;
@@ -1200,10 +1241,15 @@ atr_status atr_batch_set_config(atr_batch* b, const atr_batch_config* cfg)
     if ((cfg->encode_items != 8 && cfg->encode_items != 16) ||
         (cfg->sort_items != 8 && cfg->sort_items != 16))
         return fail(ATR_ERR_INVALID_ARGUMENT, "encode_items and sort_items must be 8 or 16");
-    if (cfg->encode_items != b->encode_items || cfg->sort_items != b->sort_items)
+    const int tb = cfg->treelet_block == 0 ? 128 : cfg->treelet_block;
+    if (tb != 64 && tb != 128 && tb != 256)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "treelet_block must be 64, 128 or 256 (0 = 128)");
+    if (cfg->encode_items != b->encode_items || cfg->sort_items != b->sort_items ||
+        tb != b->treelet_block)
         batch_drop_graph(b);
     b->encode_items = cfg->encode_items;
     b->sort_items = cfg->sort_items;
+    b->treelet_block = tb;
     return ATR_OK;
 }
 
@@ -1212,6 +1258,7 @@ atr_status atr_batch_get_config(const atr_batch* b, atr_batch_config* cfg)
     if (!b || !cfg) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     cfg->encode_items = b->encode_items;
     cfg->sort_items = b->sort_items;
+    cfg->treelet_block = b->treelet_block;
     return ATR_OK;
 }
 

@@ -272,4 +272,41 @@ __device__ __forceinline__ uint32_t block256_exclusive_sum(uint32_t v, uint32_t*
     return add + inc - v;
 }
 
+// --------------------------------------------------------- traversal stack
+// Per-lane stack of a traversal: the newest RING entries live in an LDS ring
+// laid out [entry][thread] (a wave's lanes touch consecutive words), older
+// entries spill to a private array of SPILL entries, which only deep
+// traversals touch.  `ring` points at this thread's column of the block's
+// RING * BLOCK words.  push() refuses (returns false) beyond RING + SPILL
+// entries instead of writing out of bounds.
+template <int RING, int SPILL, int BLOCK>
+struct LdsStack {
+    uint32_t* ring;
+    uint32_t spill[SPILL > 0 ? SPILL : 1];
+    int n = 0;
+
+    __device__ __forceinline__ explicit LdsStack(uint32_t* column) : ring(column) {}
+    __device__ __forceinline__ bool push(uint32_t v)
+    {
+        if (n >= RING) {
+            const int old = n - RING;
+            if (old >= SPILL) return false;
+            spill[old] = ring[(old % RING) * BLOCK];
+        }
+        ring[(n % RING) * BLOCK] = v;
+        ++n;
+        return true;
+    }
+    __device__ __forceinline__ uint32_t pop()
+    {
+        --n;
+        const uint32_t v = ring[(n % RING) * BLOCK];
+        if (n >= RING) {
+            const int old = n - RING;
+            ring[(old % RING) * BLOCK] = spill[old];
+        }
+        return v;
+    }
+};
+
 }  // namespace atr

@@ -63,7 +63,8 @@ struct atr_context {
     // events are created or recorded then either)
     bool no_alloc = false;
     // device status word: bit 0 = a bounded wait gave up, bit 1 = a tree was
-    // deeper than a traversal stack; cleared when atr_sync reports it
+    // deeper than a traversal stack, bit 2 = a traversal layout's counts were
+    // inconsistent; cleared when atr_sync reports it
     uint32_t* d_status = nullptr;
     // iterations of a bounded wait before it gives up (ATRBVH_SPIN_LIMIT
     // overrides the default at context creation)
@@ -133,6 +134,14 @@ struct atr_collapsed {
     float4* ttris = nullptr;     // 3 float4 per slot (v0, e1, e2), 2n slots
     int32_t* ttri_id = nullptr;  // 2n
     bool has_layout = false;
+    const float4* verts = nullptr;  // the bound scene's vertices (layout emission)
+    // the reference tracer's Compact / Compact2 layout (atr_collapsed_compact)
+    float4* cnodes = nullptr;       // 4 float4 per emitted node
+    float4* cwoop = nullptr;        // Woop rows
+    int32_t* crows_index = nullptr; // triangle id per Woop row
+    uint32_t compact_nodes = 0;
+    uint32_t compact_rows = 0;
+    int compact_div = 0;
     CollapseSummary* d_summary = nullptr;
     CollapseSummary h_summary{};
     int pinned_slot = -1;
@@ -172,6 +181,7 @@ struct atr_batch {
     // onesweep thread
     int encode_items = 16;
     int sort_items = 16;
+    int treelet_block = 128;  // threads per workgroup of the treelet passes
     // captured build sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -276,16 +286,18 @@ atr_status run_mesh_step(atr_context* ctx, float4* mesh, uint32_t V, float* grad
                          float max_step, const uint32_t* index, uint32_t n_tris, float4* soup);
 
 // Treelets (treelet.hip)
+// `block`: threads per workgroup of the pass (64, 128 or 256; 128 unless a
+// batch's launch shape says otherwise).
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
                            float ct, int32_t* tris, float* nsah, uint32_t* counters,
-                           int4* rec);
+                           int4* rec, int block = 128);
 // Node records {left, right, area(left), area(right)} that the ATRBVH passes
 // read while forming treelets (first_leaf entries); built before the first
 // pass of every optimisation and kept current by the passes.
 atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec);
 atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
                           int32_t* tris, float* nsah, uint32_t* counters,
-                          const int32_t* d_schedule, int rounds);
+                          const int32_t* d_schedule, int rounds, int block = 128);
 
 // Collapse + traversal layout (collapse.hip)
 // verts may be NULL: then only the reference collapsed tree is produced
@@ -294,6 +306,12 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
                         const float4* verts);
 atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32_t* out,
                         uint32_t* total);
+
+// The reference tracer's layout and its traversal (compact.hip)
+atr_status run_compact(atr_context* ctx, atr_collapsed* c, const float4* verts, int div);
+atr_status run_trace_compact(atr_context* ctx, int num_rays, int any_hit, const float4* rays,
+                             int2* results, const float4* nodes, const float4* tris,
+                             const int32_t* tri_index);
 
 // Traversal + ray generation (trace.hip)
 atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uint32_t r,

@@ -165,20 +165,23 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ rays, 
 // missed early or finished early do not idle while the rest of the wave
 // traverses.  Per ray the traversal is exactly k_trace's (same slab test,
 // near child first, far child pushed, LIFO pops), so every ray gets the same
-// result.  The most recent stack entry lives in a register: a pop returns it
-// at once and the refill load from the private array overlaps the next node
-// fetch.  The host uses this kernel only when the tree height is below 64,
+// result.  The most recent stack entry lives in a register (a pop returns it
+// at once and the refill overlaps the next node fetch); the entries below it
+// are an LdsStack: the newest 16 per lane in LDS, up to 48 more in private
+// memory.  The host uses this kernel only when the tree height is below 64,
 // which bounds the stack.
+constexpr int kFetchRing = 16;
 template <bool ANY>
 __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ rays, uint32_t R,
                                                      const float4* __restrict__ nodes,
                                                      const float4* __restrict__ tris,
                                                      const int32_t* __restrict__ tri_id,
                                                      int4* __restrict__ results,
-                                                     uint32_t* __restrict__ counter)
+                                                     uint32_t* __restrict__ counter,
+                                                     uint32_t* __restrict__ err)
 {
-    constexpr int kStack = 64;
-    int stack[kStack];
+    __shared__ uint32_t s_ring[kFetchRing * 256];
+    LdsStack<kFetchRing, 48, 256> stack(s_ring + threadIdx.x);
     const int lane = (int)__lane_id();
     const uint64_t lt = lanemask_lt();
     bool active = false;
@@ -216,6 +219,7 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
                     bu = 0.0f;
                     bv = 0.0f;
                     sp = 0;
+                    stack.n = 0;
                     node = t > tmin ? 0 : kDone;
                     active = true;
                 }
@@ -241,10 +245,15 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
                 const int c1 = __float_as_int(n3.y);
                 if (h0 && h1) {
                     const bool swap = t0 > t1;
-                    if (sp > 0) stack[sp - 1] = top;
-                    top = swap ? c0 : c1;
-                    ++sp;
-                    node = swap ? c1 : c0;
+                    if (sp > 0 && !stack.push((uint32_t)top)) {
+                        atomicOr(err, 2u);
+                        node = kDone;
+                        id = -1;
+                    } else {
+                        top = swap ? c0 : c1;
+                        ++sp;
+                        node = swap ? c1 : c0;
+                    }
                 } else if (h0) {
                     node = c0;
                 } else if (h1) {
@@ -252,7 +261,7 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
                 } else if (sp > 0) {
                     node = top;
                     --sp;
-                    if (sp > 0) top = stack[sp - 1];
+                    if (sp > 0) top = (int)stack.pop();
                 } else {
                     node = kDone;
                 }
@@ -284,7 +293,7 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
             } else {
                 node = top;
                 --sp;
-                if (sp > 0) top = stack[sp - 1];
+                if (sp > 0) top = (int)stack.pop();
             }
         }
         if (active && node == kDone) {
@@ -530,10 +539,10 @@ atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uin
             s = launch(ctx, any_hit ? "trace_any" : "trace", bytes, [&] {
                 if (any_hit)
                     k_trace_fetch<true><<<pgrid, 256, 0, ctx->stream>>>(
-                        rays, r, c->tnodes, c->ttris, c->ttri_id, out, counter);
+                        rays, r, c->tnodes, c->ttris, c->ttri_id, out, counter, ctx->d_status);
                 else
                     k_trace_fetch<false><<<pgrid, 256, 0, ctx->stream>>>(
-                        rays, r, c->tnodes, c->ttris, c->ttri_id, out, counter);
+                        rays, r, c->tnodes, c->ttris, c->ttri_id, out, counter, ctx->d_status);
             });
         dfree(ctx, counter);
         return s;

@@ -619,14 +619,28 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
 }
 
 // ------------------------------------------------------------------ driver
-// Workgroups of 128 threads (one leaf each, two waves): smaller groups free
-// their LDS sooner once most of their threads have stopped climbing, which
-// measured ~3% faster than 256 on the 2M-triangle build (64 was slower).
-constexpr int kPassBlock = 128;
-constexpr int kPassShift = 7;
+// Workgroups of BLOCK threads (one leaf each; 64, 128 or 256): smaller groups
+// free their LDS sooner once most of their threads have stopped climbing; 128
+// measured ~3% faster than 256 on the 2M-triangle build (64 was slower), and
+// batched builds let the launch-shape tuner choose.  The trees do not depend
+// on it: the block size only decides which nodes hand over through LDS.
+template <int BLOCK>
+struct PassShift;
+template <>
+struct PassShift<64> {
+    static constexpr int value = 6;
+};
+template <>
+struct PassShift<128> {
+    static constexpr int value = 7;
+};
+template <>
+struct PassShift<256> {
+    static constexpr int value = 8;
+};
 
-template <bool AGG>
-__global__ __launch_bounds__(kPassBlock) void k_treelet_pass(uint32_t n_leaves, int first_leaf, int n,
+template <bool AGG, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int first_leaf, int n,
                                                       int gamma, float ci,
                                                       float ct, int32_t* parent, int32_t* left,
                                                       int32_t* right, float4* bbmin,
@@ -636,15 +650,16 @@ __global__ __launch_bounds__(kPassBlock) void k_treelet_pass(uint32_t n_leaves, 
                                                       const int32_t* sched, int rounds,
                                                       int half_words)
 {
+    constexpr int kShift = PassShift<BLOCK>::value;
     extern __shared__ float smem[];
-    __shared__ uint32_t s_cnt[kPassBlock];
+    __shared__ uint32_t s_cnt[BLOCK];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int half = lane >> 5;
     const int hl = lane & 31;
     float* lds = smem + ((tid >> 6) * 2 + half) * half_words;
 
-    const uint32_t gid = blockIdx.x * (uint32_t)kPassBlock + (uint32_t)tid;
+    const uint32_t gid = blockIdx.x * (uint32_t)BLOCK + (uint32_t)tid;
     const int blk = (int)blockIdx.x;
     s_cnt[tid] = 0u;
     __syncthreads();
@@ -683,9 +698,9 @@ __global__ __launch_bounds__(kPassBlock) void k_treelet_pass(uint32_t n_leaves, 
             const int r = right[cur];
             const float ar = area[cur];
             up = parent[cur];
-            local = block_local<kPassShift>(sp, cur, blk);
+            local = block_local<kShift>(sp, cur, blk);
             const uint32_t old =
-                local ? arrive_local(&s_cnt[cur & (kPassBlock - 1)])
+                local ? arrive_local(&s_cnt[cur & (BLOCK - 1)])
                       : arrive_crossing(&counters[cur]);
             if (old == 0u) {
                 cur = -1;
@@ -817,34 +832,56 @@ atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec)
     });
 }
 
-atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
-                           float ct, int32_t* tris, float* nsah, uint32_t* counters, int4* rec)
+template <bool AGG, int BLOCK>
+static void launch_pass(atr_context* ctx, atr_tree* t, int n, int gamma, float ci, float ct,
+                        int32_t* tris, float* nsah, uint32_t* counters, int4* rec,
+                        const int32_t* sched, int rounds, int half_words)
 {
     const uint32_t N = t->n;
-    const int first_leaf = (int)t->first_leaf;
+    const size_t lds = (size_t)half_words * (BLOCK / 32) * sizeof(float);
+    k_treelet_pass<AGG, BLOCK><<<div_up(N, BLOCK), BLOCK, lds, ctx->stream>>>(
+        N, (int)t->first_leaf, n, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin,
+        t->bbmax, t->area, tris, nsah, counters, t->span, rec, sched, rounds, half_words);
+}
+
+template <bool AGG>
+static atr_status launch_pass_block(atr_context* ctx, const char* name, int block, atr_tree* t,
+                                    int n, int gamma, float ci, float ct, int32_t* tris,
+                                    float* nsah, uint32_t* counters, int4* rec,
+                                    const int32_t* sched, int rounds, int half_words)
+{
+    if (block != 64 && block != 128 && block != 256)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "treelet pass block must be 64, 128 or 256");
+    return launch(ctx, name, 0.0, [&] {
+        if (block == 64)
+            launch_pass<AGG, 64>(ctx, t, n, gamma, ci, ct, tris, nsah, counters, rec, sched,
+                                 rounds, half_words);
+        else if (block == 128)
+            launch_pass<AGG, 128>(ctx, t, n, gamma, ci, ct, tris, nsah, counters, rec, sched,
+                                  rounds, half_words);
+        else
+            launch_pass<AGG, 256>(ctx, t, n, gamma, ci, ct, tris, nsah, counters, rec, sched,
+                                  rounds, half_words);
+    });
+}
+
+atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
+                           float ct, int32_t* tris, float* nsah, uint32_t* counters, int4* rec,
+                           int block)
+{
     ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
     const int half_words = kAggFixedWords + treelet * (treelet - 1) / 2;
-    const size_t lds = (size_t)half_words * (kPassBlock / 32) * sizeof(float);
-    return launch(ctx, "atrbvh_pass", 0.0, [&] {
-        k_treelet_pass<true><<<div_up(N, kPassBlock), kPassBlock, lds, ctx->stream>>>(
-            N, first_leaf, treelet, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin,
-            t->bbmax, t->area, tris, nsah, counters, t->span, rec, nullptr, 0, half_words);
-    });
+    return launch_pass_block<true>(ctx, "atrbvh_pass", block, t, treelet, gamma, ci, ct, tris,
+                                   nsah, counters, rec, nullptr, 0, half_words);
 }
 
 atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
                           int32_t* tris, float* nsah, uint32_t* counters,
-                          const int32_t* d_schedule, int rounds)
+                          const int32_t* d_schedule, int rounds, int block)
 {
-    const uint32_t N = t->n;
-    const int first_leaf = (int)t->first_leaf;
     ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
-    const size_t lds = (size_t)kTrWords * (kPassBlock / 32) * sizeof(float);
-    return launch(ctx, "trbvh_pass", 0.0, [&] {
-        k_treelet_pass<false><<<div_up(N, kPassBlock), kPassBlock, lds, ctx->stream>>>(
-            N, first_leaf, kTrN, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin, t->bbmax,
-            t->area, tris, nsah, counters, t->span, nullptr, d_schedule, rounds, kTrWords);
-    });
+    return launch_pass_block<false>(ctx, "trbvh_pass", block, t, kTrN, gamma, ci, ct, tris, nsah,
+                                    counters, nullptr, d_schedule, rounds, kTrWords);
 }
 
 }  // namespace atr

@@ -208,6 +208,7 @@ def batched_leg(ctx, meshes: int, reps: int):
         t.close()
         s.close()
     b.close()
+    cpp = cpp_plain_loop(meshes, tuned["best"], reps=3)
     return {"meshes": meshes, "triangles": total,
             "graph_ms": round(graph_ms, 3), "eager_ms": round(eager_ms, 3),
             "builds_per_s": round(meshes / (graph_ms * 1e-3), 1),
@@ -217,7 +218,28 @@ def batched_leg(ctx, meshes: int, reps: int):
             "speedup_vs_plain": round(loop_ms / graph_ms, 2),
             "sah_identical": bool(same),
             "mean_sah": round(float(np.mean(sah_batch)), 4),
+            "cpp_loop": cpp,
             "autotune": tuned}
+
+
+def cpp_plain_loop(meshes: int, shape: dict, reps: int = 3):
+    """The same batch against the per-mesh loop, both timed from one C++ host
+    thread through the C ABI (atrbvh_amd/plain_loop.cpp), so that the loop
+    pays no interpreter overhead per call."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "atrbvh_amd", "plain_loop")
+    if not os.path.exists(exe):
+        return {"error": f"{exe} is not built"}
+    cmd = [exe, str(meshes), "3", "64", "65536", "3000", str(reps),
+           str(shape["encode_items"]), str(shape["sort_items"]), str(shape["treelet_block"])]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    line = (r.stdout.strip().splitlines() or ["{}"])[-1]
+    try:
+        out = json.loads(line)
+    except json.JSONDecodeError:
+        out = {"error": f"rc {r.returncode}: {r.stdout[-300:]} {r.stderr[-300:]}"}
+    return out
 
 
 def differentiable_leg(ctx, steps: int = 50, lr: float = 0.005, max_step: float = 0.002):
@@ -612,8 +634,33 @@ def main():
                  "primary_hit_rate": round(n_diffuse / r, 4),
                  "diffuse_rays": n_diffuse, "diffuse_ms": round(sec_ms, 3),
                  "diffuse_Mrays_s": round(n_diffuse / (sec_ms * 1e-3) / 1e6, 1)}
+        # the reference tracer's layout (Compact2, Woop triangles) and its
+        # `trace` ABI on the same rays
+        ctx.sync()
+        c0 = time.perf_counter()
+        lay = collapsed.compact(16)
+        layout_ms = (time.perf_counter() - c0) * 1000.0
+        d_res3 = ctx.buffer(r * 16)
+        ctx.trace_compact(r, d_prim, d_res3, lay)
+        ctx.sync()
+        w0 = time.perf_counter()
+        for _ in range(treps):
+            ctx.trace_compact(r, d_prim, d_res3, lay)
+        ctx.sync()
+        w1 = time.perf_counter()
+        for _ in range(treps):
+            ctx.trace_compact(r, d_sec, d_res3, lay)
+        ctx.sync()
+        w2 = time.perf_counter()
+        cp_ms = (w1 - w0) * 1000.0 / treps
+        cs_ms = (w2 - w1) * 1000.0 / treps
+        trace["compact_woop"] = {
+            "layout_ms": round(layout_ms, 3), "nodes": lay.num_nodes, "rows": lay.num_rows,
+            "primary_ms": round(cp_ms, 3), "primary_Mrays_s": round(r / (cp_ms * 1e-3) / 1e6, 1),
+            "diffuse_ms": round(cs_ms, 3),
+            "diffuse_Mrays_s": round(n_diffuse / (cs_ms * 1e-3) / 1e6, 1)}
         log(f"[bench] trace primary {trace['primary_Mrays_s']} Mrays/s, diffuse "
-            f"{trace['diffuse_Mrays_s']} Mrays/s")
+            f"{trace['diffuse_Mrays_s']} Mrays/s; Woop/Compact2 {trace['compact_woop']}")
     collapsed.close()
     tree.close()
 
@@ -623,7 +670,7 @@ def main():
         batched = batched_leg(ctx, args.batch_meshes, reps=3)
         log(f"[bench] batched {batched['graph_ms']} ms ({batched['builds_per_s']} builds/s), "
             f"plain loop {batched['plain_loop_ms']} ms; x{batched['speedup_vs_plain']}, "
-            f"SAH identical: {batched['sah_identical']}")
+            f"SAH identical: {batched['sah_identical']}; C++ loop {batched['cpp_loop']}")
 
     adjoint = None
     if not args.no_adjoint:

@@ -196,6 +196,43 @@ atr_status atr_collapsed_destroy(atr_collapsed* c);
 atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, int any_hit,
                      int32_t* d_results);
 
+/* The reference tracer's BVH layout, built on the GPU from a collapsed tree:
+ * CudaBVH::createCompact(const BVHCollapsedTree*, Scene*, nodeOffsetSizeDiv)
+ * (gpu-ray-traversal/src/rt/cuda/CudaBVH.cpp:469-574, layout described in
+ * CudaBVH.hpp:41-54) with Woop triangles (woopifyTri, :579-600), byte for
+ * byte what that host DFS writes.  node_offset_size_div 1 = Compact (child
+ * pointers in bytes), 16 = Compact2 (in 16-byte rows, what atr_trace_compact
+ * and kepler_dynamic_fetch consume).  The arrays are device memory owned by
+ * the collapsed tree (valid until it is destroyed or this is called again);
+ * the call blocks while it reads back the sizes.  The tree must have been
+ * bound to a scene. */
+typedef struct atr_compact_layout {
+    const float* nodes;         /* num_nodes x 16 words (4 x int4: child boxes, pointers) */
+    uint32_t num_nodes;
+    const float* tris;          /* num_rows x float4: 3 Woop rows per triangle,
+                                   a row of 0x80000000 words ending each leaf */
+    const int32_t* tri_indices; /* num_rows: triangle id on a triangle's first row */
+    uint32_t num_rows;
+    int node_offset_size_div;
+} atr_compact_layout;
+atr_status atr_collapsed_compact(atr_collapsed* c, int node_offset_size_div,
+                                 atr_compact_layout* out);
+
+/* The reference's kernel ABI for traversal (`trace`,
+ * gpu-ray-traversal/src/rt/kernels/CudaTracerKernels.hpp:73-86) as a host
+ * call on the context's stream, with kepler_dynamic_fetch semantics
+ * (rt/kernels/kepler_dynamic_fetch.cu:61-320): Compact2 nodes in nodesA, Woop
+ * rows in trisA, triIndices as written by atr_collapsed_compact; nodesB..D
+ * and trisB..C are unused (as for the reference's Compact layouts).  rays:
+ * float4[2 num_rays]; results: int4[num_rays] of which x = triangle id or -1
+ * and y = t bits are written (the padding is left untouched, as the
+ * reference's STORE_RESULT does).  A traversal deeper than 128 entries stops
+ * that ray and raises the status that atr_sync reports. */
+atr_status atr_trace_compact(atr_context* ctx, int num_rays, int any_hit, const float* rays,
+                             int32_t* results, const float* nodesA, const float* nodesB,
+                             const float* nodesC, const float* nodesD, const float* trisA,
+                             const float* trisB, const float* trisC, const int32_t* triIndices);
+
 /* Pinhole primary rays, row-major over a w x h grid: origin `eye`, direction
  * normalize(((2(x+.5)/w-1)*tan_half, (2(y+.5)/h-1)*tan_half, 1)) rotated into
  * the frame (right, up, forward) given by `basis` (9 floats, rows). */
@@ -299,11 +336,13 @@ atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float
                            int use_graph);
 /* Launch shape of batched builds (results do not depend on it): triangles per
  * thread of the encode kernel and keys per thread of a radix-sort tile, 8 or
- * 16 each (default 16/16).  Changing it drops the captured graph.  Chosen per
- * workload by atrbvh_amd.autotune. */
+ * 16 each (default 16/16), and threads per workgroup of the treelet passes,
+ * 64, 128 or 256 (0 = 128).  Changing it drops the captured graph.  Chosen
+ * per workload by atrbvh_amd.autotune. */
 typedef struct atr_batch_config {
     int32_t encode_items;
     int32_t sort_items;
+    int32_t treelet_block;
 } atr_batch_config;
 atr_status atr_batch_set_config(atr_batch* b, const atr_batch_config* cfg);
 atr_status atr_batch_get_config(const atr_batch* b, atr_batch_config* cfg);

@@ -13,6 +13,8 @@
 
 #include <stdint.h>
 
+#include "atrbvh.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -38,6 +40,26 @@ void atr_gen_batch_sizes(uint64_t seed, uint32_t meshes, uint32_t lo, uint32_t h
  * triangle offsets[m] (host threads, deterministic). */
 void atr_gen_soup_batch(uint64_t seed_base, uint32_t meshes, const uint32_t* offsets, float sigma,
                         float* vertices);
+
+/* Wavefront OBJ scenes, read as the reference framework's importer reads
+ * them (gpu-ray-traversal/src/framework/io/MeshWavefrontIO.cpp:330-545):
+ * "v" positions (exactly three numbers, parsed with the framework's fp32
+ * accumulation, String.cpp:440-491), "f" faces ("p", "p/t", "p/t/n",
+ * "p//n"; negative indices count from the end; out-of-range positions read
+ * as (0,0,0)) triangulated as fans, faces grouped by the materials of
+ * "mtllib" files and listed group by group in order of first use, as the
+ * reference's Scene lists them (rt/Scene.cpp:55-77).  Host only. */
+typedef struct atr_mesh atr_mesh;
+atr_status atr_mesh_load_obj(const char* path, atr_mesh** out);
+atr_status atr_mesh_info(const atr_mesh* mesh, uint32_t* num_triangles, uint32_t* num_positions);
+/* The triangle soup, float4[3 * num_triangles] with w = 0. */
+atr_status atr_mesh_soup(const atr_mesh* mesh, float* vertices);
+atr_status atr_mesh_destroy(atr_mesh* mesh);
+/* SceneConverter::ConvertToCudaSceneDeviceMemory
+ * (gpu-ray-traversal/src/rt/SceneConverter.cpp:19-71): the soup and its box
+ * (running maximum starting at FLT_MIN) computed on the host, then
+ * atr_scene_create. */
+atr_status atr_scene_create_from_mesh(atr_context* ctx, const atr_mesh* mesh, atr_scene** out);
 
 /* TRBVH lane schedule (the library's own generator): writes rounds*warp ints
  * into out when cap allows and returns the number of rounds, or -1. */

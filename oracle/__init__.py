@@ -345,6 +345,47 @@ class Sbvh:
             pass
 
 
+def woopify(v0, v1, v2):
+    """woopifyTri (CudaBVH.cpp:579-600): the three Woop rows, float32[3, 4]."""
+    out = np.zeros(12, np.float32)
+    a = [_f32(x).reshape(-1)[:3].copy() for x in (v0, v1, v2)]
+    lib().orc_woopify(_p(a[0], _f32p), _p(a[1], _f32p), _p(a[2], _f32p), _p(out, _f32p))
+    return out.reshape(3, 4)
+
+
+def compact(c: Collapsed, v4, div=16):
+    """createCompact (CudaBVH.cpp:469-574): (nodes int32[M, 16], woop
+    float32[R, 4], tri_index int32[R])."""
+    v4 = _f32(v4)
+    cs = c.struct()
+    m = ctypes.c_uint32()
+    r = ctypes.c_uint32()
+    if lib().orc_compact(ctypes.byref(cs), _p(v4, _f32p), div, None, ctypes.byref(m), None, None,
+                         ctypes.byref(r)) != 0:
+        raise ValueError("orc_compact failed")
+    nodes = np.zeros((m.value, 16), np.int32)
+    woop = np.zeros((r.value, 4), np.float32)
+    tri = np.zeros(r.value, np.int32)
+    lib().orc_compact(ctypes.byref(cs), _p(v4, _f32p), div, _p(nodes, _i32p), ctypes.byref(m),
+                      _p(woop, _f32p), _p(tri, _i32p), ctypes.byref(r))
+    return nodes, woop, tri
+
+
+def trace_compact(nodes, woop, tri_index, rays, any_hit=False):
+    """kepler_dynamic_fetch `trace` semantics on a Compact2 layout; int32[r, 4]
+    {id or -1, t bits, 0, 0}."""
+    nodes = np.ascontiguousarray(nodes, np.int32)
+    woop = _f32(woop)
+    tri_index = np.ascontiguousarray(tri_index, np.int32)
+    rays = _f32(rays)
+    r = rays.shape[0]
+    res = np.zeros((r, 4), np.int32)
+    lib().orc_trace_compact(_p(nodes, _i32p), _p(woop, _f32p), _p(tri_index, _i32p),
+                            _p(rays, _f32p), ctypes.c_uint32(r), ctypes.c_int(1 if any_hit else 0),
+                            _p(res, _i32p))
+    return res
+
+
 def rays_diffuse(v4, prim, res, seed, tmax=np.inf):
     """rayGenAOKernel with one sample per ray (RayGenKernels.cu:114-221)."""
     v4 = _f32(v4)

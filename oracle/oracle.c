@@ -1560,3 +1560,268 @@ void orc_rays_diffuse(const float* v4, const float* prim, const int32_t* res, ui
         q[7] = tri == -1 ? -1.0f : tmax;
     }
 }
+
+/* --- the reference tracer's Compact layout and Woop test ------------------ */
+/* 3x3 determinant of framework/base/Math.hpp:930-935 (detImpl, L = 3), in its
+ * operand order. */
+static float orc_det3(float m[3][3])
+{
+    return m[0][0] * m[1][1] * m[2][2] - m[0][0] * m[1][2] * m[2][1] +
+           m[1][0] * m[2][1] * m[0][2] - m[1][0] * m[2][2] * m[0][1] +
+           m[2][0] * m[0][1] * m[1][2] - m[2][0] * m[0][2] * m[1][1];
+}
+
+/* CudaBVH::woopifyTri (gpu-ray-traversal/src/rt/cuda/CudaBVH.cpp:579-600):
+ * the inverse of [v0 - v2 | v1 - v2 | (v0 - v2) x (v1 - v2) | v2] (last row
+ * 0 0 0 1) by MatrixBase::inverted (Math.hpp:961-980: cofactors, the
+ * determinant accumulated over every (i, j), result * rcp(d) * 4); the three
+ * rows kept are (row 2 with its last entry negated, row 0, row 1). */
+void orc_woopify(const float* v0, const float* v1, const float* v2, float out[12])
+{
+    float a[3] = {v0[0] - v2[0], v0[1] - v2[1], v0[2] - v2[2]};
+    float b[3] = {v1[0] - v2[0], v1[1] - v2[1], v1[2] - v2[2]};
+    float c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    float m[4][4];
+    for (int r = 0; r < 3; ++r) {
+        m[r][0] = a[r];
+        m[r][1] = b[r];
+        m[r][2] = c[r];
+        m[r][3] = v2[r];
+    }
+    m[3][0] = 0.0f;
+    m[3][1] = 0.0f;
+    m[3][2] = 0.0f;
+    m[3][3] = 1.0f;
+    float inv[4][4];
+    float d = 0.0f;
+    float si = 1.0f;
+    for (int i = 0; i < 4; ++i) {
+        float sj = si;
+        for (int j = 0; j < 4; ++j) {
+            float sub[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) sub[k][l] = m[k < j ? k : k + 1][l < i ? l : l + 1];
+            float dd = orc_det3(sub) * sj;
+            inv[i][j] = dd;
+            d += dd * m[j][i];
+            sj = -sj;
+        }
+        si = -si;
+    }
+    const float rd = d != 0.0f ? 1.0f / d : 0.0f;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) inv[i][j] = inv[i][j] * rd * 4.0f;
+    out[0] = inv[2][0];
+    out[1] = inv[2][1];
+    out[2] = inv[2][2];
+    out[3] = -inv[2][3];
+    for (int k = 0; k < 4; ++k) {
+        out[4 + k] = inv[0][k];
+        out[8 + k] = inv[1][k];
+    }
+}
+
+/* CudaBVH::createCompact(const BVHCollapsedTree*, Scene*, nodeOffsetSizeDiv)
+ * (CudaBVH.cpp:469-574): a host DFS from the root with an explicit stack
+ * (last in, first out); each internal node gets 4 int4 (child boxes' x/y,
+ * both z ranges, the two child pointers: an internal child's byte offset /
+ * div, ~(first triangle row) for a leaf).  A child is a leaf iff its data
+ * index is >= 0 (:504).  A leaf appends 3 Woop rows per triangle (its list,
+ * or its single triangle) and a row of 0x80000000 words; tri_index gets the
+ * triangle id and two zeros per triangle, 0 per terminator.  With NULL
+ * outputs only the sizes are returned (rows of 16 bytes each). */
+int orc_compact(const orc_collapsed* c, const float* v4, int div, int32_t* nodes,
+                uint32_t* num_nodes, float* woop, int32_t* tri_index, uint32_t* num_rows)
+{
+    const uint32_t N = c->n;
+    if (N < 2 || (div != 1 && div != 16)) return -1;
+    int32_t* stack_node = (int32_t*)malloc(sizeof(int32_t) * 2 * N);
+    int32_t* stack_slot = (int32_t*)malloc(sizeof(int32_t) * 2 * N);
+    if (!stack_node || !stack_slot) {
+        free(stack_node);
+        free(stack_slot);
+        return -1;
+    }
+    uint32_t nn = 1, rows = 0;
+    int sp = 0;
+    stack_node[sp] = c->root;
+    stack_slot[sp] = 0;
+    ++sp;
+    while (sp > 0) {
+        --sp;
+        const int index = stack_node[sp];
+        const int slot = stack_slot[sp];
+        const int ch[2] = {c->left[index], c->right[index]};
+        int32_t cidx[2];
+        for (int k = 0; k < 2; ++k) {
+            const int child = ch[k];
+            if (c->data[child] < 0) {
+                cidx[k] = (int32_t)((uint64_t)nn * 64u / (uint32_t)div);
+                stack_node[sp] = child;
+                stack_slot[sp] = (int32_t)nn;
+                ++sp;
+                ++nn;
+            } else {
+                cidx[k] = ~(int32_t)rows;
+                const int cnt = c->count[child];
+                for (int j = 0; j < (cnt > 1 ? cnt : 1); ++j) {
+                    const int tri = cnt > 1 ? c->tri_index[c->data[child] + j] : c->data[child];
+                    if (woop) {
+                        const float* a = v4 + (size_t)tri * 12;
+                        orc_woopify(a, a + 4, a + 8, woop + (size_t)rows * 4);
+                        tri_index[rows] = tri;
+                        tri_index[rows + 1] = 0;
+                        tri_index[rows + 2] = 0;
+                    }
+                    rows += 3;
+                }
+                if (woop) {
+                    const uint32_t term = 0x80000000u;
+                    for (int k2 = 0; k2 < 4; ++k2) memcpy(woop + (size_t)rows * 4 + k2, &term, 4);
+                    tri_index[rows] = 0;
+                }
+                rows += 1;
+            }
+        }
+        if (nodes) {
+            const float* lmn = c->bbmin + (size_t)ch[0] * 4;
+            const float* lmx = c->bbmax + (size_t)ch[0] * 4;
+            const float* rmn = c->bbmin + (size_t)ch[1] * 4;
+            const float* rmx = c->bbmax + (size_t)ch[1] * 4;
+            int32_t* dst = nodes + (size_t)slot * 16;
+            dst[0] = fbits(lmn[0]);
+            dst[1] = fbits(lmx[0]);
+            dst[2] = fbits(lmn[1]);
+            dst[3] = fbits(lmx[1]);
+            dst[4] = fbits(rmn[0]);
+            dst[5] = fbits(rmx[0]);
+            dst[6] = fbits(rmn[1]);
+            dst[7] = fbits(rmx[1]);
+            dst[8] = fbits(lmn[2]);
+            dst[9] = fbits(lmx[2]);
+            dst[10] = fbits(rmn[2]);
+            dst[11] = fbits(rmx[2]);
+            dst[12] = cidx[0];
+            dst[13] = cidx[1];
+            dst[14] = 0;
+            dst[15] = 0;
+        }
+    }
+    free(stack_node);
+    free(stack_slot);
+    if (num_nodes) *num_nodes = nn;
+    if (num_rows) *num_rows = rows;
+    return 0;
+}
+
+/* Integer min/max of float bit patterns, as the reference tracer's video
+ * instructions compare them (CudaTracerKernels.hpp:165-184). */
+static float orc_imin(float a, float b) { return fbits(a) < fbits(b) ? a : b; }
+static float orc_imax(float a, float b) { return fbits(a) > fbits(b) ? a : b; }
+
+/* The closest-hit / any-hit result of kepler_dynamic_fetch's `trace`
+ * (gpu-ray-traversal/src/rt/kernels/kepler_dynamic_fetch.cu:61-320) on the
+ * Compact2 layout (node pointers in 16-byte rows), for one ray at a time:
+ * slab spans of spanBeginKepler / spanEndKepler
+ * (CudaTracerKernels.hpp:209-210), the nearer child first (a tie keeps child
+ * 0), leaves in the order the traversal reaches them, and the Woop test
+ * (:257-277) with tmin < t < hitT, u >= 0, v >= 0, u + v <= 1.  Deferring a
+ * leaf, as the kernel does, only visits extra nodes with a staler hitT and
+ * leaves the result unchanged.  results int32[4r]: {id or -1, t bits, 0, 0}
+ * where id is tri_index[row of the hit]. */
+void orc_trace_compact(const int32_t* nodes, const float* woop, const int32_t* tri_index,
+                       const float* rays, uint32_t r, int any_hit, int32_t* results)
+{
+    const int32_t sentinel = 0x76543210;
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * 4096);
+    if (!stack) return;
+    for (uint32_t i = 0; i < r; ++i) {
+        const float* o = rays + 8ull * i;
+        const float* d = o + 4;
+        const float ooeps = exp2f(-80.0f);
+        const float idx = 1.0f / (fabsf(d[0]) > ooeps ? d[0] : copysignf(ooeps, d[0]));
+        const float idy = 1.0f / (fabsf(d[1]) > ooeps ? d[1] : copysignf(ooeps, d[1]));
+        const float idz = 1.0f / (fabsf(d[2]) > ooeps ? d[2] : copysignf(ooeps, d[2]));
+        const float ox = o[0] * idx, oy = o[1] * idy, oz = o[2] * idz;
+        const float tmin = o[3];
+        float hitT = d[3];
+        int hit = -1;
+        int sp = 0;
+        stack[sp] = sentinel;
+        int32_t node = 0;
+        while (node != sentinel) {
+            if (node >= 0) {
+                const int32_t* q = nodes + (size_t)node * 4;
+                float n[12];
+                memcpy(n, q, sizeof(n));
+                const float c0lox = n[0] * idx - ox, c0hix = n[1] * idx - ox;
+                const float c0loy = n[2] * idy - oy, c0hiy = n[3] * idy - oy;
+                const float c0loz = n[8] * idz - oz, c0hiz = n[9] * idz - oz;
+                const float c1loz = n[10] * idz - oz, c1hiz = n[11] * idz - oz;
+                const float c1lox = n[4] * idx - ox, c1hix = n[5] * idx - ox;
+                const float c1loy = n[6] * idy - oy, c1hiy = n[7] * idy - oy;
+                const float c0min = orc_imax(orc_imax(fminf(c0lox, c0hix), fminf(c0loy, c0hiy)),
+                                             orc_imax(orc_imin(c0loz, c0hiz), tmin));
+                const float c0max = orc_imin(orc_imin(fmaxf(c0lox, c0hix), fmaxf(c0loy, c0hiy)),
+                                             orc_imin(orc_imax(c0loz, c0hiz), hitT));
+                const float c1min = orc_imax(orc_imax(fminf(c1lox, c1hix), fminf(c1loy, c1hiy)),
+                                             orc_imax(orc_imin(c1loz, c1hiz), tmin));
+                const float c1max = orc_imin(orc_imin(fmaxf(c1lox, c1hix), fmaxf(c1loy, c1hiy)),
+                                             orc_imin(orc_imax(c1loz, c1hiz), hitT));
+                const int t0 = c0max >= c0min, t1 = c1max >= c1min;
+                int32_t a = q[12], b = q[13];
+                if (!t0 && !t1) {
+                    node = stack[sp--];
+                } else {
+                    node = t0 ? a : b;
+                    if (t0 && t1) {
+                        if (c1min < c0min) {
+                            const int32_t tmp = node;
+                            node = b;
+                            b = tmp;
+                        }
+                        stack[++sp] = b;
+                    }
+                }
+                continue;
+            }
+            /* leaf: rows ~node, ~node + 3, ... up to the terminator */
+            int stop = 0;
+            for (int row = ~node;; row += 3) {
+                const float* v00 = woop + (size_t)row * 4;
+                const float* v11 = v00 + 4;
+                const float* v22 = v00 + 8;
+                if (fbits(v00[0]) == (int32_t)0x80000000) break;
+                const float Oz = v00[3] - o[0] * v00[0] - o[1] * v00[1] - o[2] * v00[2];
+                const float invDz = 1.0f / (d[0] * v00[0] + d[1] * v00[1] + d[2] * v00[2]);
+                const float t = Oz * invDz;
+                if (t > tmin && t < hitT) {
+                    const float Ox = v11[3] + o[0] * v11[0] + o[1] * v11[1] + o[2] * v11[2];
+                    const float Dx = d[0] * v11[0] + d[1] * v11[1] + d[2] * v11[2];
+                    const float u = Ox + t * Dx;
+                    if (u >= 0.0f) {
+                        const float Oy = v22[3] + o[0] * v22[0] + o[1] * v22[1] + o[2] * v22[2];
+                        const float Dy = d[0] * v22[0] + d[1] * v22[1] + d[2] * v22[2];
+                        const float v = Oy + t * Dy;
+                        if (v >= 0.0f && u + v <= 1.0f) {
+                            hitT = t;
+                            hit = row;
+                            if (any_hit) {
+                                stop = 1;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+            if (stop) break;
+            node = stack[sp--];
+        }
+        int32_t* out = results + 4ull * i;
+        out[0] = hit < 0 ? -1 : tri_index[hit];
+        out[1] = fbits(hitT);
+        out[2] = 0;
+        out[3] = 0;
+    }
+    free(stack);
+}

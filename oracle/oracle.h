@@ -138,6 +138,17 @@ void orc_ray_inverse(const float* d, float* inv);
 int orc_ray_triangle(const float* v0, const float* v1, const float* v2, const float* o,
                      const float* d, float tmin, float tmax, float* t, float* u, float* v);
 
+/* --- the reference tracer's layout (CudaBVH.cpp:469-600) ----------------- */
+void orc_woopify(const float* v0, const float* v1, const float* v2, float out[12]);
+/* createCompact with nodeOffsetSizeDiv div (1 = Compact, 16 = Compact2); NULL
+ * outputs => sizes only.  nodes: 16 int32 per node; woop: 4 floats per row;
+ * tri_index: one int32 per row. */
+int orc_compact(const orc_collapsed* c, const float* v4, int div, int32_t* nodes,
+                uint32_t* num_nodes, float* woop, int32_t* tri_index, uint32_t* num_rows);
+/* kepler_dynamic_fetch trace semantics on a Compact2 layout. */
+void orc_trace_compact(const int32_t* nodes, const float* woop, const int32_t* tri_index,
+                       const float* rays, uint32_t r, int any_hit, int32_t* results);
+
 /* --- ray generation ----------------------------------------------------- */
 /* One diffuse ray per primary ray (rayGenAOKernel, RayGenKernels.cu:114-221,
  * numSamples = 1); prim float4[2r], res int32[4r] as orc_trace writes them. */

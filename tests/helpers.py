@@ -98,8 +98,10 @@ def camera_rays_host(g: int, eye=CAMERA_EYE) -> np.ndarray:
     return rays
 
 
-def compare_hits(dev: np.ndarray, ref: np.ndarray, rtol: float = 1e-6):
-    """Identical hit ids except exact-distance ties; t within rtol."""
+def compare_hits(dev: np.ndarray, ref: np.ndarray, rtol: float = 1e-6, atol: float = 0.0):
+    """Identical hit ids except exact-distance ties; t within rtol (plus atol,
+    for comparisons of two different hit tests whose t carry absolute
+    rounding of the scene's coordinates)."""
     dev = np.asarray(dev).reshape(-1, 4)
     ref = np.asarray(ref).reshape(-1, 4)
     td = dev[:, 1].copy().view(np.float32)
@@ -107,10 +109,10 @@ def compare_hits(dev: np.ndarray, ref: np.ndarray, rtol: float = 1e-6):
     hit_r = ref[:, 0] >= 0
     assert np.array_equal(dev[:, 0] >= 0, hit_r), "hit/miss pattern differs"
     same = dev[:, 0] == ref[:, 0]
-    tie = ~same & np.isclose(td, tr, rtol=rtol, atol=0)
+    tie = ~same & np.isclose(td, tr, rtol=rtol, atol=atol)
     assert np.all(same | tie), f"{np.count_nonzero(~(same | tie))} rays hit different triangles"
     h = hit_r
-    assert np.allclose(td[h], tr[h], rtol=rtol, atol=0)
+    assert np.allclose(td[h], tr[h], rtol=rtol, atol=atol)
     return int(np.count_nonzero(~same))
 
 

@@ -221,13 +221,14 @@ def test_autotuner_launch_shapes_keep_trees(ctx):
     b.build(9, 2, graph=False)
     ref = [b.extract(m).download() for m in (0, 5, 29)]
     res = tune_launch(ctx, b, reps=1)
-    assert len(res["table"]) == 4 and b.config() == {k: res["best"][k] for k in
-                                                     ("encode_items", "sort_items")}
-    for enc, srt in ((8, 8), (16, 8), (8, 16)):
-        b.set_config(enc, srt)
+    assert len(res["table"]) == 12 and b.config() == {k: res["best"][k] for k in
+                                                      ("encode_items", "sort_items",
+                                                       "treelet_block")}
+    for enc, srt, blk in ((8, 8, 64), (16, 8, 256), (8, 16, 128)):
+        b.set_config(enc, srt, blk)
         b.build(9, 2, graph=True)
         for m, r in zip((0, 5, 29), ref):
-            assert_tree_equal(b.extract(m).download(), r, f"shape{enc}/{srt}.mesh{m}")
+            assert_tree_equal(b.extract(m).download(), r, f"shape{enc}/{srt}/{blk}.mesh{m}")
     tr = tune_treelet(ctx, b, sah_tol=0.05, reps=1)
     assert tr["best"]["mean_sah"] <= tr["sah_limit"]
     b.close()

@@ -121,7 +121,8 @@ def test_c3_full_batch_against_oracle_and_plain_path(ctx):
     assert recs["onesweep64"]["launches"] == 6, recs.get("onesweep64")
     eager = b.sah()
     tuned = tune_launch(ctx, b, 9, 2, reps=1)
-    assert b.config() == {k: tuned["best"][k] for k in ("encode_items", "sort_items")}
+    assert b.config() == {k: tuned["best"][k] for k in ("encode_items", "sort_items",
+                                                        "treelet_block")}
     b.build(9, 2, graph=True)
     sah = b.sah()
     assert np.array_equal(bits(sah), bits(eager))
