@@ -88,6 +88,11 @@ def lib():
         L.atr_scene_vertices.argtypes = [_vp, ctypes.POINTER(_vp)]
         L.atr_scene_destroy.argtypes = [_vp]
         L.atr_lbvh_build.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.atr_rays_morton_sort.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp, _vp]
+        L.atr_rays_morton_keys.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, _vp]
+        L.atr_rays_unsort_results.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.atr_lbvh_build_boxes.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int,
+                                           ctypes.POINTER(_vp)]
         L.atr_trbvh_optimize.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                          ctypes.c_float]
         L.atr_atrbvh_optimize.argtypes = L.atr_trbvh_optimize.argtypes
@@ -380,6 +385,24 @@ class Context:
         b = np.asarray(basis, np.float32).reshape(9)
         check(lib().atr_rays_pinhole(self.h, _ptr(e), _ptr(b), tan_half, w, h, tmax, d_rays.ptr))
 
+    def rays_morton_sort(self, d_rays: DeviceBuffer, n: int, d_out: DeviceBuffer,
+                         d_slot_to_id: DeviceBuffer, d_id_to_slot: DeviceBuffer | None = None,
+                         d_slot_to_id_in: DeviceBuffer | None = None):
+        """RayBuffer::mortonSort (gpu-ray-traversal/src/rt/ray/RayBuffer.cpp:103-163)."""
+        check(lib().atr_rays_morton_sort(self.h, d_rays.ptr, n,
+                                         d_slot_to_id_in.ptr if d_slot_to_id_in else None,
+                                         d_out.ptr, d_slot_to_id.ptr,
+                                         d_id_to_slot.ptr if d_id_to_slot else None))
+
+    def rays_morton_keys(self, d_rays: DeviceBuffer, n: int, d_keys: DeviceBuffer,
+                         d_box: DeviceBuffer):
+        check(lib().atr_rays_morton_keys(self.h, d_rays.ptr, n, d_keys.ptr, d_box.ptr))
+
+    def rays_unsort_results(self, d_sorted: DeviceBuffer, d_slot_to_id: DeviceBuffer, n: int,
+                            d_out: DeviceBuffer):
+        check(lib().atr_rays_unsort_results(self.h, d_sorted.ptr, d_slot_to_id.ptr, n,
+                                            d_out.ptr))
+
     def rays_diffuse(self, scene: "Scene", d_primary: DeviceBuffer, d_results: DeviceBuffer,
                      n: int, seed: int, tmax: float, d_rays: DeviceBuffer):
         check(lib().atr_rays_diffuse(self.h, scene.h, d_primary.ptr, d_results.ptr, n, seed, tmax,
@@ -649,6 +672,15 @@ class LBVHBuilder:
         h = _vp()
         check(lib().atr_lbvh_build(scene.h, self.bits, ctypes.byref(h)))
         return Tree(scene.ctx, h.value, scene.n)
+
+    def BuildTreeFromBoxes(self, scene: Scene, d_bbmin: DeviceBuffer, d_bbmax: DeviceBuffer,
+                           d_tri: DeviceBuffer, m: int) -> Tree:
+        """The triangle-splitting build (LBVH.cu:99-161, 320-428, 572-669) over m
+        device boxes (float4 rows), each naming a triangle of `scene`."""
+        h = _vp()
+        check(lib().atr_lbvh_build_boxes(scene.h, d_bbmin.ptr, d_bbmax.ptr, d_tri.ptr, m,
+                                         self.bits, ctypes.byref(h)))
+        return Tree(scene.ctx, h.value, m)
 
 
 class TRBVHOptimizer:

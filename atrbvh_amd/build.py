@@ -40,7 +40,7 @@ DEVICE = [
 ]
 
 SOURCES = ["lbvh.hip", "treelet.hip", "collapse.hip", "trace.hip", "batch.hip", "adjoint.hip",
-           "noalloc.hip", "compact.hip", "api.cpp", "obj.cpp",
+           "noalloc.hip", "compact.hip", "raysort.hip", "api.cpp", "obj.cpp",
            "scenes.cpp", "schedule.cpp"]
 
 # Per-file flags.  treelet.hip only ever handles finite boxes, areas and SAH

@@ -490,9 +490,67 @@ atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out)
     return ATR_OK;
 }
 
+atr_status atr_lbvh_build_boxes(atr_scene* scene, const float* d_bbmin, const float* d_bbmax,
+                                const int32_t* d_tri, uint32_t num_boxes, int morton_bits,
+                                atr_tree** out)
+{
+    if (!scene || !d_bbmin || !d_bbmax || !d_tri || !out)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
+    if (num_boxes < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "at least 2 boxes are needed");
+    if (num_boxes > (1u << 30)) return fail(ATR_ERR_INVALID_ARGUMENT, "too many boxes");
+    atr_context* ctx = scene->ctx;
+    const uint32_t n = num_boxes;
+    const int kb = morton_bits == 63 ? 8 : 4;
+    const int passes = morton_bits == 63 ? 8 : 4;
+    atr_tree* t = nullptr;
+    ATR_TRY(tree_alloc(ctx, n, &t));
+    void* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;
+    uint64_t* slots = nullptr;
+    const uint32_t blocks = std::max(encode_blocks(n), 1u);
+    atr_status s = ATR_OK;
+    if (s == ATR_OK) s = dalloc(ctx, (size_t)n * kb, &keys[0]);
+    if (s == ATR_OK) s = dalloc(ctx, (size_t)n * kb, &keys[1]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &vals[0]);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &vals[1]);
+    if (s == ATR_OK) s = dalloc_t(ctx, (size_t)blocks * passes * 256, &hist);
+    if (s == ATR_OK) s = dalloc_t(ctx, n - 1, &slots);
+    const float4* bmn = (const float4*)d_bbmin;
+    const float4* bmx = (const float4*)d_bbmax;
+    if (s == ATR_OK)
+        s = run_encode_boxes(ctx, bmn, bmx, n, scene->d_bbox, morton_bits, keys[0], hist, blocks,
+                             slots, n - 1);
+    bool in_alt = false;
+    if (s == ATR_OK)
+        s = run_sort(ctx, kb * 8, passes, keys[0], vals[0], n, keys[1], vals[1], true, hist,
+                     blocks, nullptr, &in_alt);
+    if (s == ATR_OK)
+        s = run_build_boxes(ctx, morton_bits, keys[in_alt ? 1 : 0], vals[in_alt ? 1 : 0], n, bmn,
+                            bmx, d_tri, t, slots);
+    dfree(ctx, keys[0]);
+    dfree(ctx, keys[1]);
+    dfree(ctx, vals[0]);
+    dfree(ctx, vals[1]);
+    dfree(ctx, hist);
+    dfree(ctx, slots);
+    if (s != ATR_OK) {
+        atr_tree_destroy(t);
+        return s;
+    }
+    t->scene = scene;
+    t->box_leaves = true;
+    *out = t;
+    return ATR_OK;
+}
+
 atr_status atr_tree_bind_scene(atr_tree* t, atr_scene* scene)
 {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
+    if (t->box_leaves)
+        return fail(ATR_ERR_UNSUPPORTED, "a tree built from primitive boxes keeps its scene");
     if (scene && scene->n != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "size mismatch");
     t->scene = scene;
     return ATR_OK;
@@ -932,6 +990,41 @@ atr_status atr_rays_diffuse(atr_context* ctx, atr_scene* scene, const float* d_p
                             num_rays, seed, tmax, (float4*)d_rays);
 }
 
+atr_status atr_rays_morton_sort(atr_context* ctx, const float* d_rays, uint32_t num_rays,
+                                const int32_t* d_slot_to_id_in, float* d_rays_out,
+                                int32_t* d_slot_to_id, int32_t* d_id_to_slot)
+{
+    if (!ctx || !d_rays || !d_rays_out || !d_slot_to_id)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (d_rays_out == d_rays) return fail(ATR_ERR_INVALID_ARGUMENT, "output overlaps input");
+    if (num_rays == 0) return ATR_OK;
+    if (num_rays > (1u << 30)) return fail(ATR_ERR_INVALID_ARGUMENT, "too many rays");
+    return run_ray_sort(ctx, (const float4*)d_rays, num_rays, d_slot_to_id_in,
+                        (float4*)d_rays_out, d_slot_to_id, d_id_to_slot);
+}
+
+atr_status atr_rays_morton_keys(atr_context* ctx, const float* d_rays, uint32_t num_rays,
+                                uint64_t* d_keys, float* d_box)
+{
+    if (!ctx || !d_rays || !d_keys || !d_box)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (num_rays == 0) return ATR_OK;
+    return run_ray_keys(ctx, (const float4*)d_rays, num_rays, d_keys, d_box);
+}
+
+atr_status atr_rays_unsort_results(atr_context* ctx, const int32_t* d_sorted_results,
+                                   const int32_t* d_slot_to_id, uint32_t num_rays,
+                                   int32_t* d_results)
+{
+    if (!ctx || !d_sorted_results || !d_slot_to_id || !d_results)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (d_results == d_sorted_results)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "output overlaps input");
+    if (num_rays == 0) return ATR_OK;
+    return run_unsort_results(ctx, (const int4*)d_sorted_results, d_slot_to_id, num_rays,
+                              (int4*)d_results);
+}
+
 // ------------------------------------------------------------------ stages
 atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t n,
                              const float bbmin[3], const float bbmax[3], int morton_bits,
@@ -988,6 +1081,9 @@ atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t
 atr_status atr_refit(atr_tree* t, atr_scene* scene)
 {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (t->box_leaves)
+        return fail(ATR_ERR_UNSUPPORTED,
+                    "a tree built from primitive boxes cannot be refit from triangle vertices");
     if (!scene) scene = t->scene;
     if (!scene) return fail(ATR_ERR_INVALID_ARGUMENT, "no scene given or bound to the tree");
     if (scene->n != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "scene/tree size mismatch");

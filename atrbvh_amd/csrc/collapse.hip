@@ -201,9 +201,11 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
 
 // Right-first DFS of one child subtree into its list (writeSubtreeLeavesNonrecursive,
 // TreeCollapser.cu:95-137, the pending stack living at the end of the list's
-// own slots).  Every tree leaf holds a distinct triangle, so the reference's
-// duplicate check never removes anything and is not repeated.
-__device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off,
+// own slots).  A triangle already in the list is not written again
+// (:112-125); the list's count is then the number of distinct triangles.
+// Trees built from triangles have one leaf per triangle, so the check is
+// made only for trees built from primitive boxes (`dedup`).
+__device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off, bool dedup,
                           const int32_t* __restrict__ left, const int32_t* __restrict__ right,
                           const int32_t* __restrict__ data, const float4* __restrict__ v,
                           int32_t* c_data, int32_t* c_count, int32_t* tri_index, float4* ttris,
@@ -219,18 +221,23 @@ __device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off,
         --ss;
         if (idx >= (int)N - 1) {
             const int tri = data[idx];
-            buf[pos] = tri;
-            if (v) {
-            const uint32_t slot = N + off + (uint32_t)pos;
-            const float4 a = v[3ull * tri];
-            const float4 b = v[3ull * tri + 1];
-            const float4 c = v[3ull * tri + 2];
-            ttris[3ull * slot] = a;
-            ttris[3ull * slot + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
-            ttris[3ull * slot + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
-            ttri_id[slot] = pos == cnt - 1 ? (int32_t)((uint32_t)tri | 0x80000000u) : tri;
+            bool fresh = true;
+            if (dedup)
+                for (int q = 0; q < pos && fresh; ++q) fresh = buf[q] != tri;
+            if (fresh) {
+                buf[pos] = tri;
+                if (v) {
+                    const uint32_t slot = N + off + (uint32_t)pos;
+                    const float4 a = v[3ull * tri];
+                    const float4 b = v[3ull * tri + 1];
+                    const float4 c = v[3ull * tri + 2];
+                    ttris[3ull * slot] = a;
+                    ttris[3ull * slot + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+                    ttris[3ull * slot + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+                    ttri_id[slot] = tri;
+                }
+                ++pos;
             }
-            ++pos;
         } else {
             const int l = left[idx];
             const int r = right[idx];
@@ -238,13 +245,15 @@ __device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off,
             buf[cnt - 1 - ss++] = r;
         }
     }
+    // the list's last triangle carries the end mark
+    if (v) ttri_id[N + off + (uint32_t)pos - 1] |= (int32_t)0x80000000u;
     c_data[child] = (int32_t)off;
     c_count[child] = pos;
 }
 
 __global__ __launch_bounds__(256) void k_collapse_emit(
-    uint32_t N, const uint32_t* __restrict__ emit_count, const uint32_t* __restrict__ emit_off,
-    const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+    uint32_t N, bool dedup, const uint32_t* __restrict__ emit_count,
+    const uint32_t* __restrict__ emit_off, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
     const int32_t* __restrict__ data, const float4* __restrict__ v, int32_t* c_data,
     int32_t* c_count, int32_t* tri_index, float4* ttris, int32_t* ttri_id)
 {
@@ -257,12 +266,12 @@ __global__ __launch_bounds__(256) void k_collapse_emit(
     const int cl = c_count[l];
     const int cr = c_count[r];
     if (cl > 1) {
-        emit_list(N, l, cl, off, left, right, data, v, c_data, c_count, tri_index, ttris,
+        emit_list(N, l, cl, off, dedup, left, right, data, v, c_data, c_count, tri_index, ttris,
                   ttri_id);
         off += (uint32_t)cl;
     }
     if (cr > 1)
-        emit_list(N, r, cr, off, left, right, data, v, c_data, c_count, tri_index, ttris,
+        emit_list(N, r, cr, off, dedup, left, right, data, v, c_data, c_count, tri_index, ttris,
                   ttri_id);
 }
 
@@ -372,7 +381,8 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
     }
     ATR_TRY(launch(ctx, "collapse_emit", 0.0, [&] {
         k_collapse_emit<<<div_up(N, 256), 256, 0, ctx->stream>>>(
-            N, emit_count, emit_off, t->left, t->right, t->data, verts, c->data, c->count,
+            N, t->box_leaves, emit_count, emit_off, t->left, t->right, t->data, verts, c->data,
+            c->count,
             c->tri_index, c->ttris, c->ttri_id);
     }));
     if (verts) {

@@ -96,11 +96,16 @@ __global__ __launch_bounds__(256) void k_compact_pos(uint32_t N, const int32_t* 
     pos_of[x] = pos;
 }
 
+// Leaves of a collapsed tree are read by node kind: a tree leaf (index
+// >= N-1) holds its triangle in its data index, a collapsed internal node a
+// list of count[] triangles at that offset.  The reference's createCompact
+// tells them apart by triangleCount > 1 (CudaBVH.cpp:514-533), which agrees
+// except for a list that the collapse's duplicate removal (trees built from
+// split triangles) reduced to one triangle; that list is read as a list.
 __device__ __forceinline__ int leaf_rows(int child, uint32_t N, const int32_t* count)
 {
     if (child >= (int)N - 1) return 4;
-    const int c = count[child];
-    return 3 * (c > 1 ? c : 1) + 1;
+    return 3 * count[child] + 1;
 }
 
 __device__ __forceinline__ bool is_inner(int child, uint32_t N, const int32_t* data)
@@ -238,7 +243,7 @@ __global__ __launch_bounds__(256) void k_compact_emit(
             ++kslot;
         } else {
             cidx[k] = ~(int)row;
-            const bool list = c < (int)N - 1 && count[c] > 1;
+            const bool list = c < (int)N - 1;
             const int cnt = list ? count[c] : 1;
             for (int j = 0; j < cnt; ++j) {
                 const int tri = list ? tri_index[data[c] + j] : data[c];

@@ -108,6 +108,10 @@ struct atr_tree {
     // node; a node whose span and index lie in one workgroup's block of leaves
     // (256, or 128 in the treelet passes) is handled by that workgroup only
     int2* span = nullptr;
+    // built from primitive boxes (atr_lbvh_build_boxes): several leaves may
+    // name one triangle, so collapsed leaf lists drop repeats, and a refit
+    // from the scene's vertices would not give the leaves' boxes
+    bool box_leaves = false;
 };
 
 // Summary written by the collapse kernels into device memory.
@@ -257,6 +261,14 @@ atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32
 uint32_t encode_blocks(uint32_t n);
 atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
                      uint32_t n, const float4* verts, atr_tree* t, uint64_t* slots);
+// Builds over caller-given primitive boxes (split triangles): keys from the
+// box centroids; leaves take the box and triangle of their routing index.
+atr_status run_encode_boxes(atr_context* ctx, const float4* bmin, const float4* bmax, uint32_t n,
+                            const float* d_bbox, int bits, void* keys, uint32_t* block_hist,
+                            uint32_t hist_blocks, uint64_t* fill64, uint32_t fill64_count);
+atr_status run_build_boxes(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
+                           uint32_t n, const float4* bmin, const float4* bmax,
+                           const int32_t* tri, atr_tree* t, uint64_t* slots);
 atr_status run_refit(atr_context* ctx, atr_tree* t, const float4* verts, uint32_t* counters);
 // Host: spans of a tree given as host arrays (post-order over the topology).
 void host_spans(uint32_t n, const int32_t* left, const int32_t* right, int2* span);
@@ -321,6 +333,16 @@ atr_status run_rays_pinhole(atr_context* ctx, const float* eye, const float* bas
 atr_status run_rays_diffuse(atr_context* ctx, const float4* verts, const float4* primary,
                             const int32_t* results, uint32_t r, uint32_t seed, float tmax,
                             float4* rays);
+
+// Ray coherence sort (raysort.hip): 6D Morton keys (three 64-bit words per
+// ray, word w of ray i at keys[w*r + i]) in the rays' box (6 floats).
+atr_status run_ray_keys(atr_context* ctx, const float4* rays, uint32_t r, uint64_t* keys,
+                        float* d_box);
+atr_status run_ray_sort(atr_context* ctx, const float4* rays, uint32_t r,
+                        const int32_t* in_slot_to_id, float4* out, int32_t* slot_to_id,
+                        int32_t* id_to_slot);
+atr_status run_unsort_results(atr_context* ctx, const int4* sorted, const int32_t* slot_to_id,
+                              uint32_t r, int4* out);
 
 // No-allocation entry points (noalloc.hip): the device address of this
 // device's scratch array, looked up once per device.

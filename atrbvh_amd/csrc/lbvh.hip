@@ -100,14 +100,54 @@ struct MortonTraits<uint64_t> {
     }
 };
 
+// Leaf primitives of a build.  TriLeaves: the triangles of a scene, each
+// leaf box the box of the three vertices (calculateLeafBoundingBox,
+// LBVH.cu:430-446) and each leaf's data index the triangle.  BoxLeaves:
+// caller-given primitive boxes, such as the fragments of split triangles,
+// each naming the triangle it belongs to (the triangle-splitting variants
+// generateMortonCodes*TSKernel LBVH.cu:99-161, buildTreeTSKernel :320-428
+// and calculateNodeBoundingBoxesTSKernel :572-669): the leaf box is the
+// given box (w dropped), the data index the given triangle.  Primitive p is
+// the input position; the sort payload (the reference's routing index)
+// carries it to the leaf.  Both give the key of the box centroid
+// (max + min) * 0.5 (getTriangleCentroid :14-40, getBoundingBoxCentroid
+// :42-51; x / 2 and x * 0.5 round identically).
+struct TriLeaves {
+    const float4* v;
+    __device__ __forceinline__ void box(uint32_t p, float4& mn, float4& mx) const
+    {
+        const float4 a = v[3ull * p];
+        const float4 b = v[3ull * p + 1];
+        const float4 c = v[3ull * p + 2];
+        mn = make_float4(fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y),
+                         fminf(fminf(a.z, b.z), c.z), 0.0f);
+        mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
+                         fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
+    }
+    __device__ __forceinline__ int32_t data(uint32_t p) const { return (int32_t)p; }
+};
+struct BoxLeaves {
+    const float4* bmin;
+    const float4* bmax;
+    const int32_t* tri;
+    __device__ __forceinline__ void box(uint32_t p, float4& mn, float4& mx) const
+    {
+        const float4 a = bmin[p];
+        const float4 b = bmax[p];
+        mn = make_float4(a.x, a.y, a.z, 0.0f);
+        mx = make_float4(b.x, b.y, b.z, 0.0f);
+    }
+    __device__ __forceinline__ int32_t data(uint32_t p) const { return tri[p]; }
+};
+
 // One thread per triangle, kEncItems triangles per thread.  Each wave keeps
 // a private LDS histogram of every 8-bit digit of every sort pass; the block
 // total is written to block_hist[block][pass][digit] (no global atomics).
 // The same launch fills scratch the later stages need (slot words to ~0,
 // counters to 0) so no separate memset is enqueued.
-template <typename K>
+template <typename K, typename Src>
 __global__ __launch_bounds__(kEncBlock) void k_encode(
-    const float4* __restrict__ v, uint32_t n, const float* __restrict__ bbox, K* __restrict__ keys,
+    Src src, uint32_t n, const float* __restrict__ bbox, K* __restrict__ keys,
     uint32_t* __restrict__ idx, uint32_t* __restrict__ block_hist, uint64_t* __restrict__ fill64,
     uint32_t fill64_count, uint32_t* __restrict__ zero32, uint32_t zero32_count)
 {
@@ -126,13 +166,12 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(
     for (int j = 0; j < kEncItems; ++j) {
         const uint32_t i = base + (uint32_t)j * kEncBlock + (uint32_t)tid;
         if (i < n) {
-            const float4 a = v[3ull * i];
-            const float4 b = v[3ull * i + 1];
-            const float4 c = v[3ull * i + 2];
-            // triangle AABB centroid (LBVH.cu:14-40), normalised (Commons.cuh:426-434)
-            float cx = (fmaxf(fmaxf(a.x, b.x), c.x) + fminf(fminf(a.x, b.x), c.x)) * 0.5f;
-            float cy = (fmaxf(fmaxf(a.y, b.y), c.y) + fminf(fminf(a.y, b.y), c.y)) * 0.5f;
-            float cz = (fmaxf(fmaxf(a.z, b.z), c.z) + fminf(fminf(a.z, b.z), c.z)) * 0.5f;
+            // primitive box centroid (LBVH.cu:14-51), normalised (Commons.cuh:426-434)
+            float4 bmn, bmx;
+            src.box(i, bmn, bmx);
+            const float cx = (bmx.x + bmn.x) * 0.5f;
+            const float cy = (bmx.y + bmn.y) * 0.5f;
+            const float cz = (bmx.z + bmn.z) * 0.5f;
             const K key = MortonTraits<K>::code((cx - bminx) / ex, (cy - bminy) / ey,
                                                 (cz - bminz) / ez);
             keys[i] = key;
@@ -629,10 +668,10 @@ struct ForestKeys {
 // it is a left child, at lo when it is a right child, and a root (both ends at
 // walls) at lo -- position 0 for a single tree, the mesh's first position in a
 // forest.  Leaf of sorted position i is node first_leaf + i.
-template <typename KV>
+template <typename KV, typename Src>
 __global__ __launch_bounds__(256) void k_build(KV kv_in, const uint32_t* __restrict__ idx,
                                                uint32_t n, int first_leaf,
-                                               const float4* __restrict__ v, int32_t* parent,
+                                               Src src, int32_t* parent,
                                                int32_t* left, int32_t* right, int32_t* data,
                                                float4* bbmin, float4* bbmax, float* area,
                                                int2* span, uint64_t* slots,
@@ -698,17 +737,12 @@ __global__ __launch_bounds__(256) void k_build(KV kv_in, const uint32_t* __restr
 
     const uint32_t i = (uint32_t)base + (uint32_t)tid;
     if (i < n) {
-    const uint32_t tri = keys.fb ? i : idx[i];
-    const float4 a = v[3ull * tri];
-    const float4 b = v[3ull * tri + 1];
-    const float4 c = v[3ull * tri + 2];
-    // calculateLeafBoundingBox (LBVH.cu:430-446)
-    float4 mn = make_float4(fminf(fminf(a.x, b.x), c.x), fminf(fminf(a.y, b.y), c.y),
-                            fminf(fminf(a.z, b.z), c.z), 0.0f);
-    float4 mx = make_float4(fmaxf(fmaxf(a.x, b.x), c.x), fmaxf(fmaxf(a.y, b.y), c.y),
-                            fmaxf(fmaxf(a.z, b.z), c.z), 0.0f);
+    // the primitive at sorted position i (its routing index) and its box
+    const uint32_t prim = keys.fb ? i : idx[i];
+    float4 mn, mx;
+    src.box(prim, mn, mx);
     int node = first_leaf + (int)i;
-    data[node] = (int32_t)tri;
+    data[node] = src.data(prim);
     left[node] = -1;
     right[node] = -1;
     area[node] = box_area4(mn, mx);
@@ -909,17 +943,41 @@ atr_status run_encode(atr_context* ctx, const float4* verts, uint32_t n, const f
         return fail(ATR_ERR_INTERNAL, "encode histogram block count mismatch");
     const int kb = bits == 63 ? 8 : 4;
     const double bytes = (double)n * (48.0 + kb + (idx ? 4.0 : 0.0));
+    const TriLeaves src{verts};
     if (bits == 63) {
         return launch(ctx, "encode63", bytes, [&] {
             k_encode<uint64_t><<<blocks, kEncBlock, 0, ctx->stream>>>(
-                verts, n, d_bbox, (uint64_t*)keys, idx, block_hist, fill64, fill64_count, zero32,
+                src, n, d_bbox, (uint64_t*)keys, idx, block_hist, fill64, fill64_count, zero32,
                 zero32_count);
         });
     }
     return launch(ctx, "encode30", bytes, [&] {
         k_encode<uint32_t><<<blocks, kEncBlock, 0, ctx->stream>>>(
-            verts, n, d_bbox, (uint32_t*)keys, idx, block_hist, fill64, fill64_count, zero32,
+            src, n, d_bbox, (uint32_t*)keys, idx, block_hist, fill64, fill64_count, zero32,
             zero32_count);
+    });
+}
+
+atr_status run_encode_boxes(atr_context* ctx, const float4* bmin, const float4* bmax, uint32_t n,
+                            const float* d_bbox, int bits, void* keys, uint32_t* block_hist,
+                            uint32_t hist_blocks, uint64_t* fill64, uint32_t fill64_count)
+{
+    const uint32_t blocks = std::max(encode_blocks(n), 1u);
+    if (hist_blocks != blocks) return fail(ATR_ERR_INTERNAL, "encode histogram block count mismatch");
+    const int kb = bits == 63 ? 8 : 4;
+    const BoxLeaves src{bmin, bmax, nullptr};
+    const double bytes = (double)n * (32.0 + kb);
+    if (bits == 63) {
+        return launch(ctx, "encode_boxes63", bytes, [&] {
+            k_encode<uint64_t><<<blocks, kEncBlock, 0, ctx->stream>>>(
+                src, n, d_bbox, (uint64_t*)keys, nullptr, block_hist, fill64, fill64_count,
+                nullptr, 0);
+        });
+    }
+    return launch(ctx, "encode_boxes30", bytes, [&] {
+        k_encode<uint32_t><<<blocks, kEncBlock, 0, ctx->stream>>>(
+            src, n, d_bbox, (uint32_t*)keys, nullptr, block_hist, fill64, fill64_count, nullptr,
+            0);
     });
 }
 
@@ -1093,17 +1151,43 @@ atr_status run_build(atr_context* ctx, int bits, const void* keys, const uint32_
     const uint32_t grid = div_up(n, 256);
     const double bytes = (double)n * 176.0;
     const int fl = (int)t->first_leaf;
+    const TriLeaves src{verts};
     if (bits == 63) {
         SingleKeys<uint64_t> kv{(const uint64_t*)keys, (int)n - 1};
         return launch(ctx, "build_refit63", bytes, [&] {
-            k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, verts, t->parent, t->left,
+            k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, src, t->parent, t->left,
                                                    t->right, t->data, t->bbmin, t->bbmax, t->area,
                                                    t->span, slots, ctx->d_status);
         });
     }
     SingleKeys<uint32_t> kv{(const uint32_t*)keys, (int)n - 1};
     return launch(ctx, "build_refit30", bytes, [&] {
-        k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, verts, t->parent, t->left,
+        k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, src, t->parent, t->left,
+                                               t->right, t->data, t->bbmin, t->bbmax, t->area,
+                                               t->span, slots, ctx->d_status);
+    });
+}
+
+atr_status run_build_boxes(atr_context* ctx, int bits, const void* keys, const uint32_t* idx,
+                           uint32_t n, const float4* bmin, const float4* bmax,
+                           const int32_t* tri, atr_tree* t, uint64_t* slots)
+{
+    const uint32_t grid = div_up(n, 256);
+    // leaf: 4 routing + 32 box + 4 triangle id read; the rest as a triangle build
+    const double bytes = (double)n * 164.0;
+    const int fl = (int)t->first_leaf;
+    const BoxLeaves src{bmin, bmax, tri};
+    if (bits == 63) {
+        SingleKeys<uint64_t> kv{(const uint64_t*)keys, (int)n - 1};
+        return launch(ctx, "build_boxes63", bytes, [&] {
+            k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, src, t->parent, t->left,
+                                                   t->right, t->data, t->bbmin, t->bbmax, t->area,
+                                                   t->span, slots, ctx->d_status);
+        });
+    }
+    SingleKeys<uint32_t> kv{(const uint32_t*)keys, (int)n - 1};
+    return launch(ctx, "build_boxes30", bytes, [&] {
+        k_build<<<grid, 256, 0, ctx->stream>>>(kv, idx, n, fl, src, t->parent, t->left,
                                                t->right, t->data, t->bbmin, t->bbmax, t->area,
                                                t->span, slots, ctx->d_status);
     });
@@ -1117,7 +1201,7 @@ atr_status run_build_forest(atr_context* ctx, const uint64_t* keys, const uint32
     ForestKeys kv{keys, (int)n - 1, d_seg, meshes};
     return launch(ctx, "build_refit_forest", (double)n * 176.0, [&] {
         k_build<<<div_up(n, 256), 256, 0, ctx->stream>>>(
-            kv, idx, n, (int)t->first_leaf, verts, t->parent, t->left, t->right, t->data,
+            kv, idx, n, (int)t->first_leaf, TriLeaves{verts}, t->parent, t->left, t->right, t->data,
             t->bbmin, t->bbmax, t->area, t->span, slots, ctx->d_status);
     });
 }

@@ -125,6 +125,26 @@ atr_status atr_scene_destroy(atr_scene* scene);
  * tree, bottom-up AABB refit.  morton_bits: 30 (lbvh) or 63 (lbvh64). */
 atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out);
 
+/* The triangle-splitting LBVH (BVHRT-Core/src/LBVH.h:36-70,96-155):
+ * DeviceGenerateMortonCodes over primitive boxes (generateMortonCodesTSKernel /
+ * generateMortonCodes64TSKernel, LBVH.cu:99-161), DeviceSort of the keys with
+ * their routing indices, DeviceBuildTree with routing indices
+ * (buildTreeTSKernel, LBVH.cu:320-428) and
+ * DeviceCalculateInternalNodeBoundingBoxes (calculateNodeBoundingBoxesTSKernel,
+ * LBVH.cu:572-669).  Input: num_boxes device boxes (float4 min / max, w
+ * ignored), such as the fragments of split triangles, and for each the index
+ * of the scene triangle it belongs to (device int32).  Keys come from the box
+ * centroids in the scene's box; leaf i (sorted order) gets the box of its
+ * routing index and that box's triangle as its data index, so several leaves
+ * may name one triangle.  The collapse of such a tree writes each triangle
+ * once per leaf list (TreeCollapser.cu:112-125).  The tree stays bound to
+ * `scene` (its vertices serve collapse and traversal); atr_refit and
+ * atr_tree_bind_scene reject it.  The arrays must stay valid until the work
+ * queued on the context's stream has run. */
+atr_status atr_lbvh_build_boxes(atr_scene* scene, const float* d_bbmin, const float* d_bbmax,
+                                const int32_t* d_tri, uint32_t num_boxes, int morton_bits,
+                                atr_tree** out);
+
 /* TRBVHOptimizer(treeletSize, iterations).Optimize
  * (BVHRT-Core/src/TRBVHOptimizer.h:27, TRBVHOptimizer.cpp:21-128).
  * treelet_size must be 7. */
@@ -245,6 +265,34 @@ atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float ba
 atr_status atr_rays_diffuse(atr_context* ctx, atr_scene* scene, const float* d_primary,
                             const int32_t* d_primary_results, uint32_t num_rays,
                             uint32_t seed, float tmax, float* d_rays);
+
+/* RayBuffer::mortonSort (gpu-ray-traversal/src/rt/ray/RayBuffer.cpp:103-163;
+ * findAABBKernel, genMortonKeysKernel, reorderRaysKernel,
+ * rt/ray/RayBufferKernels.cu:64-196): sorts num_rays rays (float4[2] each)
+ * by their 192-bit 6D Morton key -- origin in the rays' box (origins and end
+ * points o + d*tmax; end points that are not finite are left out),
+ * quantised to 2^24 steps, and (normalize(d) + 1)/2 quantised to 2^21 steps,
+ * bit i of coordinate k at key bit k + 6i -- stably (equal keys keep slot
+ * order; the reference sorts them on the host in no fixed order), all on the
+ * GPU.  d_rays_out[i] is the ray of old slot p(i); d_slot_to_id[i] =
+ * d_slot_to_id_in[p(i)] (p(i) when d_slot_to_id_in is NULL) and, when
+ * d_id_to_slot is not NULL, d_id_to_slot[d_slot_to_id[i]] = i.  The keys use
+ * IEEE division and square root (the reference builds with fast math).
+ * d_rays_out must not overlap d_rays. */
+atr_status atr_rays_morton_sort(atr_context* ctx, const float* d_rays, uint32_t num_rays,
+                                const int32_t* d_slot_to_id_in, float* d_rays_out,
+                                int32_t* d_slot_to_id, int32_t* d_id_to_slot);
+/* The keys of atr_rays_morton_sort (genMortonKeysKernel): word w (key bits
+ * 64w .. 64w+63) of ray i at d_keys[w * num_rays + i]; the rays' box (lo xyz,
+ * hi xyz) in d_box (6 device floats). */
+atr_status atr_rays_morton_keys(atr_context* ctx, const float* d_rays, uint32_t num_rays,
+                                uint64_t* d_keys, float* d_box);
+/* Results of sorted rays back in ray-id order: d_results[d_slot_to_id[s]] =
+ * d_sorted_results[s] (int4 per ray; the lookup RayBuffer keeps in its
+ * slot/id maps, RayBuffer.hpp). */
+atr_status atr_rays_unsort_results(atr_context* ctx, const int32_t* d_sorted_results,
+                                   const int32_t* d_slot_to_id, uint32_t num_rays,
+                                   int32_t* d_results);
 
 /* ------------------------------------------------------- low-level stages */
 /* DeviceGenerateMortonCodes (BVHRT-Core/src/LBVH.h:20-50): keys (u32 for 30,

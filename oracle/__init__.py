@@ -244,6 +244,49 @@ def lbvh(v4, bbmin, bbmax, bits) -> Tree:
     return t
 
 
+def lbvh_boxes(bmin4, bmax4, tri, bbmin, bbmax, bits) -> Tree:
+    """LBVH over m primitive boxes (float4 rows) with the triangle of each."""
+    bmin4 = _f32(bmin4).reshape(-1, 4)
+    bmax4 = _f32(bmax4).reshape(-1, 4)
+    tri = np.ascontiguousarray(tri, dtype=np.int32)
+    m = bmin4.shape[0]
+    t = Tree(m)
+    s = t.struct()
+    bbmin = _f32(bbmin)
+    bbmax = _f32(bbmax)
+    rc = lib().orc_lbvh_boxes(_p(bmin4, _f32p), _p(bmax4, _f32p), _p(tri, _i32p),
+                              ctypes.c_uint32(m), _p(bbmin, _f32p), _p(bbmax, _f32p),
+                              ctypes.c_int(bits), ctypes.byref(s))
+    if rc != 0:
+        raise ValueError("orc_lbvh_boxes failed")
+    t.pull(s)
+    return t
+
+
+def ray_box(rays):
+    """findAABBKernel (RayBufferKernels.cu:64-135): (lo, hi) of the rays."""
+    rays = _f32(rays).reshape(-1, 8)
+    box = np.zeros(6, np.float32)
+    lib().orc_ray_box(_p(rays, _f32p), ctypes.c_uint32(rays.shape[0]), _p(box, _f32p))
+    return box
+
+
+def ray_keys(rays, box=None):
+    """genMortonKeysKernel (RayBufferKernels.cu:139-171): (3, r) uint64 key words."""
+    rays = _f32(rays).reshape(-1, 8)
+    r = rays.shape[0]
+    box = ray_box(rays) if box is None else _f32(box)
+    keys = np.zeros((3, r), np.uint64)
+    lib().orc_ray_keys(_p(rays, _f32p), ctypes.c_uint32(r), _p(box, _f32p), _p(keys, _u64p))
+    return keys
+
+
+def ray_sort_order(rays):
+    """Old slot of every sorted slot: stable ascending order of the keys."""
+    k = ray_keys(rays)
+    return np.lexsort((k[0], k[1], k[2])).astype(np.uint32)
+
+
 def sah(t: Tree, ci=1.2, ct=1.0) -> float:
     s = t.struct()
     return float(lib().orc_sah(ctypes.byref(s), ci, ct))

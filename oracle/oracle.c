@@ -409,6 +409,89 @@ int orc_lbvh(const float* v4, uint32_t n, const float bbmin[3], const float bbma
     return 0;
 }
 
+/* The triangle-splitting LBVH over primitive boxes (LBVH.cu): keys of the
+ * box centroids (generateMortonCodesTSKernel / generateMortonCodes64TSKernel
+ * :99-161, getBoundingBoxCentroid :42-51) with identity routing indices,
+ * the stable sort, buildTreeTSKernel (:320-428: leaf data index =
+ * tri[routing]) and calculateNodeBoundingBoxesTSKernel (:572-669: leaf box =
+ * the routed box; w set to 0 here).  bmin4/bmax4 are float4[m]. */
+typedef struct {
+    orc_tree* t;
+    const float* bmin4;
+    const float* bmax4;
+} box_refit_ctx;
+
+static void box_refit_visit(void* p, int32_t node)
+{
+    box_refit_ctx* c = (box_refit_ctx*)p;
+    orc_tree* t = c->t;
+    float* mn = t->bbmin + (size_t)node * 4;
+    float* mx = t->bbmax + (size_t)node * 4;
+    if (is_leaf(node, t->n)) {
+        /* data holds the routing index until the build remaps it */
+        const float* a = c->bmin4 + (size_t)t->data[node] * 4;
+        const float* b = c->bmax4 + (size_t)t->data[node] * 4;
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = a[k];
+            mx[k] = b[k];
+        }
+        t->left[node] = -1;
+        t->right[node] = -1;
+    } else {
+        const float* lmn = t->bbmin + (size_t)t->left[node] * 4;
+        const float* lmx = t->bbmax + (size_t)t->left[node] * 4;
+        const float* rmn = t->bbmin + (size_t)t->right[node] * 4;
+        const float* rmx = t->bbmax + (size_t)t->right[node] * 4;
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = ORC_MIN(lmn[k], rmn[k]);
+            mx[k] = ORC_MAX(lmx[k], rmx[k]);
+        }
+    }
+    mn[3] = 0.0f;
+    mx[3] = 0.0f;
+    t->area[node] = box_area(mn, mx);
+}
+
+int orc_lbvh_boxes(const float* bmin4, const float* bmax4, const int32_t* tri, uint32_t m,
+                   const float bbmin[3], const float bbmax[3], int morton_bits, orc_tree* t)
+{
+    if (m < 2 || (morton_bits != 30 && morton_bits != 63)) return -1;
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * m);
+    void* k = malloc((morton_bits == 63 ? 8 : 4) * (size_t)m);
+    if (!idx || !k) {
+        free(idx);
+        free(k);
+        return -1;
+    }
+    t->n = m;
+    for (uint32_t i = 0; i < m; ++i) {
+        float c[3];
+        for (int a = 0; a < 3; ++a) {
+            const float cen = (bmin4[(size_t)i * 4 + a] + bmax4[(size_t)i * 4 + a]) / 2.0f;
+            c[a] = (cen - bbmin[a]) / (bbmax[a] - bbmin[a]);
+        }
+        if (morton_bits == 30) ((uint32_t*)k)[i] = orc_morton30(c[0], c[1], c[2]);
+        else ((uint64_t*)k)[i] = orc_morton63(c[0], c[1], c[2]);
+        idx[i] = i;
+    }
+    if (morton_bits == 30) {
+        orc_sort_pairs32((uint32_t*)k, idx, m);
+        orc_karras32((uint32_t*)k, idx, m, t);
+    } else {
+        orc_sort_pairs64((uint64_t*)k, idx, m);
+        orc_karras64((uint64_t*)k, idx, m, t);
+    }
+    free(k);
+    free(idx);
+    box_refit_ctx c = {t, bmin4, bmax4};
+    if (postorder(t, &c, box_refit_visit) != 0) return -1;
+    for (uint32_t i = 0; i < m; ++i) {
+        const size_t leaf = (size_t)m - 1 + i;
+        t->data[leaf] = tri[t->data[leaf]];
+    }
+    return 0;
+}
+
 /* SoABVHTree::SAH (SoABVHTree.cpp:50-74): serial float sums, root first,
  * then the other internal nodes in index order, then leaves. */
 float orc_sah(const orc_tree* t, float ci, float ct)
@@ -1359,14 +1442,16 @@ static void leaf_test(trace_ctx* x, int tri)
 
 /* BVH::traceRecursive (gpu-ray-traversal/src/rt/bvh/BVH.cpp:89-163) over the
  * collapsed tree: a child is a leaf iff its data index is >= 0
- * (CudaBVH.cpp:504); both children tested, nearer entry visited first. */
+ * (CudaBVH.cpp:504); both children tested, nearer entry visited first.  A
+ * collapsed internal node holds a list of count triangles at its data index,
+ * a tree leaf its triangle (read by node kind; see orc_compact). */
 static void trace_rec(trace_ctx* x, int node)
 {
     const orc_collapsed* c = x->c;
     if (x->done) return;
     if (c->data[node] >= 0) {
         int cnt = c->count[node];
-        if (cnt > 1) {
+        if (node < (int)c->n - 1) {
             for (int j = 0; j < cnt && !x->done; ++j) leaf_test(x, c->tri_index[c->data[node] + j]);
         } else {
             leaf_test(x, c->data[node]);
@@ -1663,9 +1748,13 @@ int orc_compact(const orc_collapsed* c, const float* v4, int div, int32_t* nodes
                 ++nn;
             } else {
                 cidx[k] = ~(int32_t)rows;
-                const int cnt = c->count[child];
-                for (int j = 0; j < (cnt > 1 ? cnt : 1); ++j) {
-                    const int tri = cnt > 1 ? c->tri_index[c->data[child] + j] : c->data[child];
+                /* the reference tests triangleCount > 1 (CudaBVH.cpp:514);
+                 * by node kind here, which differs only for a list that
+                 * duplicate removal reduced to one triangle */
+                const int list = child < (int)c->n - 1;
+                const int cnt = list ? c->count[child] : 1;
+                for (int j = 0; j < cnt; ++j) {
+                    const int tri = list ? c->tri_index[c->data[child] + j] : c->data[child];
                     if (woop) {
                         const float* a = v4 + (size_t)tri * 12;
                         orc_woopify(a, a + 4, a + 8, woop + (size_t)rows * 4);
@@ -1824,4 +1913,67 @@ void orc_trace_compact(const int32_t* nodes, const float* woop, const int32_t* t
         out[3] = 0;
     }
     free(stack);
+}
+
+/* --------------------------------------------------------------------- */
+/* Ray coherence sort: RayBuffer::mortonSort (gpu-ray-traversal/src/rt/ray/
+ * RayBuffer.cpp:103-163).  findAABBKernel (RayBufferKernels.cu:64-135): box of
+ * the origins and end points o + d*tmax (end points that are not finite are
+ * left out).  genMortonKeysKernel (:139-171): a = (o - lo)/(hi - lo),
+ * b = (normalize(d) + 1)*0.5, quantised by 2^24 and 2^21 with saturating
+ * conversion, bit i of coordinate k at key bit k + 6i (collectBits :57-62);
+ * IEEE division and square root.  keys: word w of ray i at keys[w*r + i]. */
+static uint32_t sat_u32(float x)
+{
+    if (!(x >= 1.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+void orc_ray_box(const float* rays, uint32_t r, float box[6])
+{
+    for (int a = 0; a < 3; ++a) {
+        box[a] = FLT_MAX;
+        box[3 + a] = -FLT_MAX;
+    }
+    for (uint32_t i = 0; i < r; ++i) {
+        const float* o = rays + (size_t)i * 8;
+        const float* d = o + 4;
+        float e[3];
+        int fin = 1;
+        for (int a = 0; a < 3; ++a) {
+            box[a] = ORC_MIN(box[a], o[a]);
+            box[3 + a] = ORC_MAX(box[3 + a], o[a]);
+            e[a] = o[a] + d[a] * d[3];
+            fin = fin && isfinite(e[a]);
+        }
+        if (fin)
+            for (int a = 0; a < 3; ++a) {
+                box[a] = ORC_MIN(box[a], e[a]);
+                box[3 + a] = ORC_MAX(box[3 + a], e[a]);
+            }
+    }
+}
+
+void orc_ray_keys(const float* rays, uint32_t r, const float box[6], uint64_t* keys)
+{
+    for (uint32_t i = 0; i < r; ++i) {
+        const float* o = rays + (size_t)i * 8;
+        float d[3] = {o[4], o[5], o[6]};
+        orc_normalize3(d);
+        uint32_t q[6];
+        for (int a = 0; a < 3; ++a) {
+            const float av = (o[a] - box[a]) / (box[3 + a] - box[a]);
+            q[a] = sat_u32(av * 256.0f * 65536.0f);
+            const float bv = (d[a] + 1.0f) * 0.5f;
+            q[3 + a] = sat_u32(bv * 32.0f * 65536.0f);
+        }
+        uint64_t w[3] = {0, 0, 0};
+        for (int k = 0; k < 6; ++k)
+            for (int b = 0; b < 32; ++b) {
+                const int pos = k + 6 * b;
+                w[pos >> 6] |= (uint64_t)((q[k] >> b) & 1u) << (pos & 63);
+            }
+        for (int a = 0; a < 3; ++a) keys[(size_t)a * r + i] = w[a];
+    }
 }

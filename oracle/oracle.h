@@ -92,6 +92,11 @@ void orc_refit(const float* v4, orc_tree* t);
 /* encode + sort + karras + refit; morton_bits 30 or 63 */
 int orc_lbvh(const float* v4, uint32_t n, const float bbmin[3], const float bbmax[3],
              int morton_bits, orc_tree* t);
+/* LBVH over primitive boxes (float4 min/max, m of them), each naming its
+ * triangle in tri[]: the triangle-splitting variant (LBVH.cu:99-161,
+ * 320-428, 572-669). */
+int orc_lbvh_boxes(const float* bmin4, const float* bmax4, const int32_t* tri, uint32_t m,
+                   const float bbmin[3], const float bbmax[3], int morton_bits, orc_tree* t);
 float orc_sah(const orc_tree* t, float ci, float ct);
 
 /* --- triangular matrix helpers (TriangularMatrix.cuh:34-52) ------------ */
@@ -171,6 +176,12 @@ void* orc_sbvh_build(const float* v4, uint32_t n, int threads, orc_sbvh_stats* s
 void orc_sbvh_trace(const void* bvh, const float* v4, const float* rays, uint32_t r, int threads,
                     int32_t* results);
 void orc_sbvh_free(void* bvh);
+
+/* --- ray coherence sort (RayBuffer.cpp:103-163, RayBufferKernels.cu:64-196)
+ * rays: float[8] each; box: lo xyz, hi xyz; keys: word w of ray i at
+ * keys[w*r + i] (three 64-bit words). */
+void orc_ray_box(const float* rays, uint32_t r, float box[6]);
+void orc_ray_keys(const float* rays, uint32_t r, const float box[6], uint64_t* keys);
 
 #ifdef __cplusplus
 }

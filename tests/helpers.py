@@ -76,7 +76,7 @@ def collapsed_leaf_sets(c: dict, n: int):
         node = stack.pop()
         for ch in (left[node], right[node]):
             if data[ch] >= 0:
-                if count[ch] > 1:
+                if ch < n - 1:  # a collapsed subtree's list
                     out[int(ch)] = tuple(int(x) for x in tri[data[ch]: data[ch] + count[ch]])
                 else:
                     out[int(ch)] = (int(data[ch]),)
@@ -114,6 +114,119 @@ def compare_hits(dev: np.ndarray, ref: np.ndarray, rtol: float = 1e-6, atol: flo
     h = hit_r
     assert np.allclose(td[h], tr[h], rtol=rtol, atol=atol)
     return int(np.count_nonzero(~same))
+
+
+def _edge_distance(v: np.ndarray, tri: int, ray: np.ndarray) -> float:
+    """World-space distance, in float64, from the point where the ray's line
+    crosses triangle tri's plane to the nearest edge of the triangle,
+    relative to 1 + the distance of that point from the origin."""
+    p = v.reshape(-1, 3, 4)[tri, :, :3].astype(np.float64)
+    o = ray[0:3].astype(np.float64)
+    d = ray[4:7].astype(np.float64)
+    nrm = np.cross(p[1] - p[0], p[2] - p[0])
+    den = d @ nrm
+    if den == 0.0:
+        return 0.0
+    x = o + ((p[0] - o) @ nrm) / den * d
+    dist = np.inf
+    for k in range(3):
+        a, b = p[k], p[(k + 1) % 3]
+        e = b - a
+        s = np.clip((x - a) @ e / (e @ e), 0.0, 1.0)
+        dist = min(dist, float(np.linalg.norm(x - (a + s * e))))
+    return dist / (1.0 + float(np.linalg.norm(x)))
+
+
+def compare_hits_two_tests(a: np.ndarray, b: np.ndarray, v: np.ndarray, rays: np.ndarray,
+                           rtol: float = 1e-5, atol: float = 1e-6, edge_eps: float = 1e-5,
+                           max_frac: float = 1e-3) -> int:
+    """Hits of two different ray/triangle tests on the same tree (the Woop
+    unit-triangle test and Moller-Trumbore) agree except where a ray grazes a
+    triangle edge: identical ids or exact-distance ties elsewhere, t within
+    rtol + atol, and on every ray where one test reports a nearer hit than the
+    other, the ray crosses that nearer triangle's plane within edge_eps
+    (relative, float64) of one of its edges.  Such rays are at most max_frac
+    of all rays.
+    Returns their number."""
+    a = np.asarray(a).reshape(-1, 4)
+    b = np.asarray(b).reshape(-1, 4)
+    ta = np.where(a[:, 0] >= 0, a[:, 1].copy().view(np.float32), np.float32(np.inf))
+    tb = np.where(b[:, 0] >= 0, b[:, 1].copy().view(np.float32), np.float32(np.inf))
+    same = a[:, 0] == b[:, 0]
+    both = (a[:, 0] >= 0) & (b[:, 0] >= 0)
+    close = both & np.isclose(ta, tb, rtol=rtol, atol=atol)
+    assert np.allclose(ta[same & both], tb[same & both], rtol=rtol, atol=atol)
+    odd = np.nonzero(~same & ~close)[0]
+    for r in odd:
+        tri = int(a[r, 0]) if ta[r] < tb[r] else int(b[r, 0])
+        m = _edge_distance(v, tri, rays[r])
+        assert m <= edge_eps, f"ray {r}: tests disagree away from an edge (distance {m:.3g})"
+    assert len(odd) <= max(3, int(max_frac * len(a))), f"{len(odd)} edge disagreements"
+    return int(len(odd))
+
+
+def _clip(poly: np.ndarray, axis: int, c: float, keep_below: bool) -> np.ndarray:
+    """Sutherland-Hodgman clip of a convex polygon (k x 3, float64) to one side
+    of the plane x[axis] = c."""
+    out = []
+    k = len(poly)
+    for i in range(k):
+        a, b = poly[i], poly[(i + 1) % k]
+        ina = a[axis] <= c if keep_below else a[axis] >= c
+        inb = b[axis] <= c if keep_below else b[axis] >= c
+        if ina:
+            out.append(a)
+        if ina != inb:
+            s = (c - a[axis]) / (b[axis] - a[axis])
+            p = a + s * (b - a)
+            p[axis] = c
+            out.append(p)
+    return np.array(out)
+
+
+def split_fragments(v: np.ndarray, every: int = 3, levels: int = 1):
+    """Triangle fragments for the triangle-splitting build: triangle i with
+    i % every == 0 is cut `levels` times, each piece at the midpoint of its
+    box's longest axis; a fragment's box is the box of its exactly clipped
+    polygon (float64), rounded outwards to float32 so the fragments' boxes
+    cover the triangle.  Other triangles keep one box.  Returns (bbmin,
+    bbmax) as float4 rows (w = 0) and the triangle of each fragment."""
+    tris = np.asarray(v, np.float32).reshape(-1, 3, 4)[:, :, :3].astype(np.float64)
+    mins, maxs, ids = [], [], []
+    for i, t in enumerate(tris):
+        pieces = [t]
+        if i % every == 0:
+            for _ in range(levels):
+                nxt = []
+                for p in pieces:
+                    lo, hi = p.min(0), p.max(0)
+                    ax = int(np.argmax(hi - lo))
+                    c = 0.5 * (lo[ax] + hi[ax])
+                    for below in (True, False):
+                        q = _clip(p, ax, c, below)
+                        if len(q) >= 3:
+                            nxt.append(q)
+                pieces = nxt
+        for p in pieces:
+            lo = p.min(0).astype(np.float32)
+            hi = p.max(0).astype(np.float32)
+            lo = np.where(lo.astype(np.float64) > p.min(0), np.nextafter(lo, np.float32(-np.inf)), lo)
+            hi = np.where(hi.astype(np.float64) < p.max(0), np.nextafter(hi, np.float32(np.inf)), hi)
+            mins.append(np.r_[lo, 0.0])
+            maxs.append(np.r_[hi, 0.0])
+            ids.append(i)
+    return (np.array(mins, np.float32), np.array(maxs, np.float32), np.array(ids, np.int32))
+
+
+def triangle_boxes(v: np.ndarray):
+    """Each triangle's own box as float4 rows (w = 0) and the identity ids."""
+    t = np.asarray(v, np.float32).reshape(-1, 3, 4)[:, :, :3]
+    n = t.shape[0]
+    mn = np.zeros((n, 4), np.float32)
+    mx = np.zeros((n, 4), np.float32)
+    mn[:, :3] = t.min(1)
+    mx[:, :3] = t.max(1)
+    return mn, mx, np.arange(n, dtype=np.int32)
 
 
 COMPAT_SRC = "tests/compat_main.cpp"
