@@ -634,6 +634,36 @@ def main():
                  "primary_hit_rate": round(n_diffuse / r, 4),
                  "diffuse_rays": n_diffuse, "diffuse_ms": round(sec_ms, 3),
                  "diffuse_Mrays_s": round(n_diffuse / (sec_ms * 1e-3) / 1e6, 1)}
+        # secondary rays sorted by their 6D Morton key first (the reference's
+        # sortSecondary option, Renderer.cpp:353-354, RayBuffer::mortonSort),
+        # sort timed apart as the reference times trace kernels alone
+        d_sorted = ctx.buffer(r * 32)
+        d_s2i = ctx.buffer(r * 4)
+        d_res4 = ctx.buffer(r * 16)
+        ctx.rays_morton_sort(d_sec, r, d_sorted, d_s2i)
+        ctx.sync()
+        w0 = time.perf_counter()
+        for _ in range(treps):
+            ctx.rays_morton_sort(d_sec, r, d_sorted, d_s2i)
+        ctx.sync()
+        w1 = time.perf_counter()
+        for _ in range(treps):
+            collapsed.trace(d_sorted, r, d_res4)
+        ctx.sync()
+        w2 = time.perf_counter()
+        sort_ms = (w1 - w0) * 1000.0 / treps
+        ssec_ms = (w2 - w1) * 1000.0 / treps
+        d_back = ctx.buffer(r * 16)
+        ctx.rays_unsort_results(d_res4, d_s2i, r, d_back)
+        same = bool(np.array_equal(d_back.download(np.int32, (r, 4))[:, :2],
+                                   d_res2.download(np.int32, (r, 4))[:, :2]))
+        trace["diffuse_sorted"] = {
+            "sort_ms": round(sort_ms, 3), "trace_ms": round(ssec_ms, 3),
+            "Mrays_s": round(n_diffuse / (ssec_ms * 1e-3) / 1e6, 1),
+            "Mrays_s_with_sort": round(n_diffuse / ((ssec_ms + sort_ms) * 1e-3) / 1e6, 1),
+            "hits_equal_unsorted": same}
+        for b in (d_sorted, d_s2i, d_res4, d_back):
+            b.free()
         # the reference tracer's layout (Compact2, Woop triangles) and its
         # `trace` ABI on the same rays
         ctx.sync()
@@ -660,7 +690,8 @@ def main():
             "diffuse_ms": round(cs_ms, 3),
             "diffuse_Mrays_s": round(n_diffuse / (cs_ms * 1e-3) / 1e6, 1)}
         log(f"[bench] trace primary {trace['primary_Mrays_s']} Mrays/s, diffuse "
-            f"{trace['diffuse_Mrays_s']} Mrays/s; Woop/Compact2 {trace['compact_woop']}")
+            f"{trace['diffuse_Mrays_s']} Mrays/s; sorted {trace['diffuse_sorted']}; "
+            f"Woop/Compact2 {trace['compact_woop']}")
     collapsed.close()
     tree.close()
 
