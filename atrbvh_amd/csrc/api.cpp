@@ -561,17 +561,15 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int itera
 {
     atr_context* ctx = t->ctx;
     const uint32_t n = t->n;
-    int32_t* tris = nullptr;
     float* nsah = nullptr;
-    uint32_t* counters = nullptr;
+    uint64_t* slots = nullptr;
     int4* rec = nullptr;
     int32_t* d_sched = nullptr;
     int rounds = 0;
     atr_status s = ATR_OK;
-    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n - 1, &tris);
     if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n - 1, &nsah);
-    if (s == ATR_OK) s = dalloc_t(ctx, n, &counters);
-    if (s == ATR_OK && agg && iterations > 0) {
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &slots);
+    if (s == ATR_OK && iterations > 0) {
         s = dalloc_t(ctx, t->first_leaf, &rec);
         if (s == ATR_OK) s = run_make_records(ctx, t, rec);
     }
@@ -590,14 +588,13 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int itera
     int gamma = treelet;
     for (int it = 0; it < iterations && s == ATR_OK; ++it) {
         if (agg)
-            s = run_atrbvh_pass(ctx, t, treelet, gamma, ci, ct, tris, nsah, counters, rec);
+            s = run_atrbvh_pass(ctx, t, treelet, gamma, ci, ct, nsah, slots, rec);
         else
-            s = run_trbvh_pass(ctx, t, gamma, ci, ct, tris, nsah, counters, d_sched, rounds);
+            s = run_trbvh_pass(ctx, t, gamma, ci, ct, nsah, slots, rec, d_sched, rounds);
         gamma *= 2;
     }
-    dfree(ctx, tris);
     dfree(ctx, nsah);
-    dfree(ctx, counters);
+    dfree(ctx, slots);
     dfree(ctx, rec);
     dfree(ctx, d_sched);
     return s;
@@ -1182,7 +1179,7 @@ static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_
     if (s == ATR_OK) s = dalloc_t(ctx, n, &b->slots);
     if (s == ATR_OK) s = dalloc_t(ctx, (size_t)b->passes * 256, &b->sort_offsets);
     if (s == ATR_OK) s = dalloc_t(ctx, sort_status_words(n, b->passes), &b->sort_status);
-    if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &b->tris);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &b->pass_slots);
     if (s == ATR_OK) s = dalloc_t(ctx, 2ull * n, &b->nsah);
     if (s == ATR_OK) s = dalloc_t(ctx, n, &b->counters);
     if (s == ATR_OK) s = dalloc_t(ctx, n, &b->rec);
@@ -1270,8 +1267,8 @@ static atr_status batch_enqueue(atr_batch* b, int treelet, int iterations, float
     int gamma = treelet;
     if (iterations > 0) ATR_TRY(run_make_records(ctx, b->forest, b->rec));
     for (int it = 0; it < iterations; ++it) {
-        ATR_TRY(run_atrbvh_pass(ctx, b->forest, treelet, gamma, ci, ct, b->tris, b->nsah,
-                                b->counters, b->rec, b->treelet_block));
+        ATR_TRY(run_atrbvh_pass(ctx, b->forest, treelet, gamma, ci, ct, b->nsah, b->pass_slots,
+                                b->rec, b->treelet_block));
         gamma *= 2;
     }
     return ATR_OK;
@@ -1505,7 +1502,7 @@ atr_status atr_batch_destroy(atr_batch* b)
     dfree(ctx, b->slots);
     dfree(ctx, b->sort_offsets);
     dfree(ctx, b->sort_status);
-    dfree(ctx, b->tris);
+    dfree(ctx, b->pass_slots);
     dfree(ctx, b->nsah);
     dfree(ctx, b->counters);
     dfree(ctx, b->rec);

@@ -175,7 +175,7 @@ struct atr_batch {
     uint64_t* slots = nullptr;
     uint32_t* sort_offsets = nullptr;
     uint32_t* sort_status = nullptr;
-    int32_t* tris = nullptr;
+    uint64_t* pass_slots = nullptr;  // arrival slots of the treelet passes (n)
     float* nsah = nullptr;
     int4* rec = nullptr;           // ATRBVH node records (n)
     uint32_t* counters = nullptr;
@@ -301,15 +301,16 @@ atr_status run_mesh_step(atr_context* ctx, float4* mesh, uint32_t V, float* grad
 // `block`: threads per workgroup of the pass (64, 128 or 256; 128 unless a
 // batch's launch shape says otherwise).
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
-                           float ct, int32_t* tris, float* nsah, uint32_t* counters,
-                           int4* rec, int block = 128);
-// Node records {left, right, area(left), area(right)} that the ATRBVH passes
+                           float ct, float* nsah, uint64_t* slots, int4* rec, int block = 128);
+// Node records {left, right, area(left), area(right)} that the treelet passes
 // read while forming treelets (first_leaf entries); built before the first
 // pass of every optimisation and kept current by the passes.
 atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec);
+// slots: first_leaf 64-bit arrival slots of the passes (cleared by each pass);
+// nsah: 2n-1 node SAH values.
 atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
-                          int32_t* tris, float* nsah, uint32_t* counters,
-                          const int32_t* d_schedule, int rounds, int block = 128);
+                          float* nsah, uint64_t* slots, int4* rec, const int32_t* d_schedule,
+                          int rounds, int block = 128);
 
 // Collapse + traversal layout (collapse.hip)
 // verts may be NULL: then only the reference collapsed tree is produced

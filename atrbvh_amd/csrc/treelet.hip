@@ -100,62 +100,13 @@ struct HalfTreelet {
     float* area;        // [32]
 };
 
-// formTreelet (Treelet.cuh:57-114) on one 32-lane half.  Lane 0 of the half
-// performs the expansions; the largest stored area among the current treelet
-// leaves (real leaves count as -FLT_MAX) is found with the reference's
-// non-power-of-two halving tree.
-template <bool COH>
-__device__ __forceinline__ void form_treelet(const HalfTreelet& h, int hl, int root, int n,
-                                             int first_leaf, const int32_t* left,
-                                             const int32_t* right, const float* area)
-{
-    if (hl == 0) {
-        const int l = ld<COH>(&left[root]);
-        const int r = ld<COH>(&right[root]);
-        h.internal[0] = root;
-        h.leaf[0] = l;
-        h.leaf[1] = r;
-        h.area[0] = ld<COH>(&area[l]);
-        h.area[1] = ld<COH>(&area[r]);
-    }
-    for (int it = 0; it < n - 2; ++it) {
-        const int m = it + 2;
-        int idx = hl;
-        float a = -INFINITY;
-        if (hl < m) {
-            a = h.area[hl];
-            if (h.leaf[hl] >= first_leaf) a = -FLT_MAX;
-        }
-        int cnt = m;
-        int sh = m >> 1;
-        while (cnt > 1) {
-            const int oi = __shfl_down(idx, sh, 32);
-            const float oa = __shfl_down(a, sh, 32);
-            if (oa > a) {
-                a = oa;
-                idx = oi;
-            }
-            cnt = (cnt + 1) >> 1;
-            sh = cnt >> 1;
-        }
-        if (hl == 0) {
-            const int rep = h.leaf[idx];
-            const int l = ld<COH>(&left[rep]);
-            const int r = ld<COH>(&right[rep]);
-            h.internal[it + 1] = rep;
-            h.leaf[idx] = l;
-            h.leaf[it + 2] = r;
-            h.area[idx] = ld<COH>(&area[l]);
-            h.area[it + 2] = ld<COH>(&area[r]);
-        }
-    }
-}
-
-// formTreelet as above, reading each expanded node's children and their
-// areas with one 16-byte load of its record {left, right, area(left),
-// area(right)} (kept by the ATRBVH pass next to the SoA arrays) instead of
-// two dependent loads.  The choices are identical: the record holds the
-// values the arrays hold.
+// formTreelet (Treelet.cuh:57-114) on one 32-lane half, from 16-byte node
+// records {left, right, area(left), area(right)} that the passes keep next to
+// the SoA arrays (one load per expansion instead of two dependent ones; the
+// records hold the values the arrays hold, so the choices are the same).
+// Lane 0 of the half performs the expansions; the largest stored area among
+// the current treelet leaves (real leaves count as -FLT_MAX) is found with
+// the reference's non-power-of-two halving tree.
 template <bool COH>
 __device__ __forceinline__ void form_treelet_rec(const HalfTreelet& h, int hl, int root, int n,
                                                  int first_leaf, const int4* rec)
@@ -200,13 +151,104 @@ __device__ __forceinline__ void form_treelet_rec(const HalfTreelet& h, int hl, i
     }
 }
 
+// formTreelet as above, with the records of the root's subtree down to depth
+// kBfsDepth fetched first, one tree level per round with all of a level's
+// loads in flight together: lane p < kBfsNodes of the half holds the node at heap
+// position p (children of p at 2p+1 and 2p+2) and its record.  Expansions
+// then take the chosen node's record from its lane; an expansion below that
+// depth loads it.  The choices are identical to one load per expansion: the
+// records hold the values the arrays hold, only the order of the loads
+// changes (4 dependent rounds instead of n-1 for a balanced 9-leaf treelet).
+constexpr int kBfsDepth = 3;
+constexpr int kBfsNodes = (2 << kBfsDepth) - 1;
+template <bool COH>
+__device__ __forceinline__ void form_treelet_bfs(const HalfTreelet& h, int32_t* pos, int hl,
+                                                 int root, int n, int first_leaf,
+                                                 const int4* rec)
+{
+    const bool upper = (__lane_id() & 32u) != 0u;
+    const int hbase = upper ? 32 : 0;
+    int bn = hl == 0 ? root : -1;
+    int4 bq = make_int4(-1, -1, 0, 0);
+#pragma unroll
+    for (int d = 0; d <= kBfsDepth; ++d) {
+        const int lo = (1 << d) - 1, hi = (2 << d) - 1;
+        if (hl >= lo && hl < hi && bn >= 0 && bn < first_leaf) bq = ld<COH>(&rec[bn]);
+        if (d < kBfsDepth) {
+            // lanes of depth d+1 take their node from the parent lane's record
+            const int par = hbase + (((hl - 1) >> 1) & 31);
+            const int cx = __shfl(bq.x, par, 64);
+            const int cy = __shfl(bq.y, par, 64);
+            if (hl >= hi && hl < (4 << d) - 1) bn = ((hl - 1) & 1) ? cy : cx;
+        }
+    }
+    if (hl == 0) {
+        h.internal[0] = root;
+        h.leaf[0] = bq.x;
+        h.leaf[1] = bq.y;
+        h.area[0] = __int_as_float(bq.z);
+        h.area[1] = __int_as_float(bq.w);
+        pos[0] = 1 < kBfsNodes ? 1 : -1;
+        pos[1] = 2 < kBfsNodes ? 2 : -1;
+    }
+    for (int it = 0; it < n - 2; ++it) {
+        const int m = it + 2;
+        int idx = hl;
+        float a = -INFINITY;
+        if (hl < m) {
+            a = h.area[hl];
+            if (h.leaf[hl] >= first_leaf) a = -FLT_MAX;
+        }
+        int cnt = m;
+        int sh = m >> 1;
+        while (cnt > 1) {
+            const int oi = __shfl_down(idx, sh, 32);
+            const float oa = __shfl_down(a, sh, 32);
+            if (oa > a) {
+                a = oa;
+                idx = oi;
+            }
+            cnt = (cnt + 1) >> 1;
+            sh = cnt >> 1;
+        }
+        int p = -1;
+        int rep = -1;
+        if (hl == 0) {
+            rep = h.leaf[idx];
+            p = pos[idx];
+        }
+        // the chosen node's heap lane in each half (wave-uniform)
+        const int p0 = __builtin_amdgcn_readlane(p, 0);
+        const int p1 = __builtin_amdgcn_readlane(p, 32);
+        int4 q = make_int4(bcast_half(bq.x, p0, p1, upper), bcast_half(bq.y, p0, p1, upper),
+                           bcast_half(bq.z, p0, p1, upper), bcast_half(bq.w, p0, p1, upper));
+        if (hl == 0) {
+            if (p < 0) q = ld<COH>(&rec[rep]);
+            h.internal[it + 1] = rep;
+            h.leaf[idx] = q.x;
+            h.leaf[m] = q.y;
+            h.area[idx] = __int_as_float(q.z);
+            h.area[m] = __int_as_float(q.w);
+            const int c = 2 * p + 1;
+            pos[idx] = (p >= 0 && c < kBfsNodes) ? c : -1;
+            pos[m] = (p >= 0 && c + 1 < kBfsNodes) ? c + 1 : -1;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ ATRBVH
 // Per-half LDS words: internal 32 + leaf 32 + area 32 + boxes 32*6 + spans
 // 32*2 + matrix.
 constexpr int kAggFixedWords = 32 + 32 + 32 + 192 + 64;
 
-template <bool COH>
-__device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, float root_sah, int n,
+// Returns the SAH the root holds afterwards (the agglomerated cost when the
+// treelet is rewritten, else root_sah), the same in every lane of the half.
+// COH: device-coherent loads (another XCD may have written the subtree in
+// this launch); BFS: the breadth-first formation (fewer dependent rounds,
+// more loads: for treelets whose latency is exposed) instead of one record
+// load per expansion.  Stores are always device-coherent.
+template <bool COH, bool BFS>
+__device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, float root_sah, int n,
                                                int first_leaf, float ci, int32_t* parent,
                                                int32_t* left, int32_t* right, float4* bbmin,
                                                float4* bbmax, float* area, float* nsah,
@@ -216,7 +258,10 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
     float* box = lds + 96;
     int2* sp = (int2*)(lds + 288);
     float* D = lds + kAggFixedWords;
-    form_treelet_rec<COH>(h, hl, root, n, first_leaf, rec);
+    if constexpr (BFS)  // the box words hold the heap positions until the boxes load
+        form_treelet_bfs<COH>(h, (int32_t*)box, hl, root, n, first_leaf, rec);
+    else
+        form_treelet_rec<COH>(h, hl, root, n, first_leaf, rec);
 
     // Each lane also carries the area of its current node (`my_area`), so
     // that every rewritten node's record gets its children's areas.
@@ -366,7 +411,8 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
     }
 
     // updateTreelet (:190-212)
-    if (h.area[1] < root_sah) {
+    const float cost = h.area[1];
+    if (cost < root_sah) {
         if (hl >= 1 && hl < n) {
             const int ni = h.internal[hl - 1];
             const int rr = h.leaf[hl];
@@ -384,33 +430,60 @@ __device__ __forceinline__ void atrbvh_treelet(float* lds, int hl, int root, flo
             st_coh(&rec[ni], make_int4(node, rr, __float_as_int(rec_al), __float_as_int(rec_ar)));
         }
     }
+    return cost < root_sah ? cost : root_sah;
 }
 
 // ------------------------------------------------------------------ TRBVH
 constexpr int kTrN = 7;
 constexpr int kTrSub = 1 << kTrN;
-// internal 8 + leaf 8 + area 8 + boxes 48 + leaf spans 16 + subset boxes 768 +
-// subset spans 256 + costs 128 + masks 128 + stack 16
-constexpr int kTrWords = 8 + 8 + 8 + 48 + 16 + kTrSub * 6 + kTrSub * 2 + kTrSub + kTrSub + 16;
+// internal 8 + leaf 8 + area 8 + boxes 48 + leaf spans 16 + costs 128 +
+// masks 128 + stack 16.  Subset boxes are not kept: the DP needs only their
+// areas, and the n-2 boxes a rewrite emits are recomputed from the leaf boxes
+// (min / max are exact, so any order gives the same box).
+constexpr int kTrWords = 8 + 8 + 8 + 48 + 16 + kTrSub + kTrSub + 16;
 
-template <bool COH>
+// Box and leaf span of subset S of the treelet's leaves.
+__device__ __forceinline__ void subset_box(const float* box, const int2* lsp, int S, float* mn,
+                                           float* mx, int2& ss)
+{
+    mn[0] = mn[1] = mn[2] = FLT_MAX;
+    mx[0] = mx[1] = mx[2] = -FLT_MAX;
+    ss = make_int2(0x7fffffff, -1);
+    for (int i = 0; i < kTrN; ++i) {
+        if (S & (1 << i)) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = fminf(box[i * 6 + k], mn[k]);
+                mx[k] = fmaxf(box[i * 6 + 3 + k], mx[k]);
+            }
+            ss.x = min(ss.x, lsp[i].x);
+            ss.y = max(ss.y, lsp[i].y);
+        }
+    }
+}
+
+// Rewrites keep the node records the formation reads current: a rewritten
+// node's record names its two new children and their areas (a subset's area
+// for an internal child, the treelet leaf's area otherwise).
+template <bool COH, bool BFS>
 __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, float root_sah,
                                               int root_tris, int first_leaf, float ci, float ct,
                                               const int32_t* sched, int rounds, int32_t* parent,
                                               int32_t* left, int32_t* right, float4* bbmin,
                                               float4* bbmax, float* area, float* nsah,
-                                              int2* span)
+                                              int2* span, int4* rec)
 {
     HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 8, lds + 16};
     float* box = lds + 24;
     int2* lsp = (int2*)(box + 48);
-    float* sbox = box + 48 + 16;
-    int2* ssp = (int2*)(sbox + kTrSub * 6);
-    float* cost = sbox + kTrSub * 6 + kTrSub * 2;
+    float* cost = box + 48 + 16;
     int32_t* mask = (int32_t*)(cost + kTrSub);
     int32_t* stk = mask + kTrSub;
     // formTreelet writes at most index n-1 = 6 into the 8-entry arrays
-    form_treelet<COH>(h, hl, root, kTrN, first_leaf, left, right, area);
+    if constexpr (BFS)  // the box words hold the heap positions until the boxes load
+        form_treelet_bfs<COH>(h, (int32_t*)box, hl, root, kTrN, first_leaf, rec);
+    else
+        form_treelet_rec<COH>(h, hl, root, kTrN, first_leaf, rec);
 
     if (hl < kTrN) {
         const int node = h.leaf[hl];
@@ -428,26 +501,9 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int s = hl * 4 + j;
-        float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
-        float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        int2 ss = make_int2(0x7fffffff, -1);
-        for (int i = 0; i < kTrN; ++i) {
-            if (s & (1 << i)) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    mn[k] = fminf(box[i * 6 + k], mn[k]);
-                    mx[k] = fmaxf(box[i * 6 + 3 + k], mx[k]);
-                }
-                ss.x = min(ss.x, lsp[i].x);
-                ss.y = max(ss.y, lsp[i].y);
-            }
-        }
-        ssp[s] = ss;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            sbox[s * 6 + k] = mn[k];
-            sbox[s * 6 + 3 + k] = mx[k];
-        }
+        float mn[3], mx[3];
+        int2 ss;
+        subset_box(box, lsp, s, mn, mx, ss);
         cost[s] = box_area(mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]);
         mask[s] = 0;
     }
@@ -575,42 +631,57 @@ __device__ __forceinline__ void trbvh_treelet(float* lds, int hl, int root, floa
             const int nl = __popc(bl);
             const int nr = __popc(br);
             if (act) {
-                int child;
+                int lchild, rchild;
+                float lar, rar;
                 if (lint) {
                     const int slot = __popc(bl & ltm);
-                    child = h.internal[next + slot];
-                    const float4 mn = make_float4(sbox[P * 6 + 0], sbox[P * 6 + 1], sbox[P * 6 + 2], 0.0f);
-                    const float4 mx = make_float4(sbox[P * 6 + 3], sbox[P * 6 + 4], sbox[P * 6 + 5], 0.0f);
-                    st_coh(&bbmin[child], mn);
-                    st_coh(&bbmax[child], mx);
-                    st_coh(&area[child], box_area4(mn, mx));
-                    st_coh(&nsah[child], cost[P]);
-                    st_coh(&span[child], ssp[P]);
-                    stk[slot] = child;
+                    lchild = h.internal[next + slot];
+                    float mn[3], mx[3];
+                    int2 ss;
+                    subset_box(box, lsp, P, mn, mx, ss);
+                    const float4 mn4 = make_float4(mn[0], mn[1], mn[2], 0.0f);
+                    const float4 mx4 = make_float4(mx[0], mx[1], mx[2], 0.0f);
+                    lar = box_area4(mn4, mx4);
+                    st_coh(&bbmin[lchild], mn4);
+                    st_coh(&bbmax[lchild], mx4);
+                    st_coh(&area[lchild], lar);
+                    st_coh(&nsah[lchild], cost[P]);
+                    st_coh(&span[lchild], ss);
+                    stk[slot] = lchild;
                     stk[8 + slot] = P;
                 } else {
-                    child = h.leaf[__ffs(P) - 1];
+                    const int li = __ffs(P) - 1;
+                    lchild = h.leaf[li];
+                    lar = h.area[li];
                 }
-                st_coh(&left[sroot], child);
-                st_coh(&parent[child], sroot);
+                st_coh(&left[sroot], lchild);
+                st_coh(&parent[lchild], sroot);
                 if (rint) {
                     const int slot = nl + __popc(br & ltm);
-                    child = h.internal[next + slot];
-                    const float4 mn = make_float4(sbox[Q * 6 + 0], sbox[Q * 6 + 1], sbox[Q * 6 + 2], 0.0f);
-                    const float4 mx = make_float4(sbox[Q * 6 + 3], sbox[Q * 6 + 4], sbox[Q * 6 + 5], 0.0f);
-                    st_coh(&bbmin[child], mn);
-                    st_coh(&bbmax[child], mx);
-                    st_coh(&area[child], box_area4(mn, mx));
+                    rchild = h.internal[next + slot];
+                    float mn[3], mx[3];
+                    int2 ss;
+                    subset_box(box, lsp, Q, mn, mx, ss);
+                    const float4 mn4 = make_float4(mn[0], mn[1], mn[2], 0.0f);
+                    const float4 mx4 = make_float4(mx[0], mx[1], mx[2], 0.0f);
+                    rar = box_area4(mn4, mx4);
+                    st_coh(&bbmin[rchild], mn4);
+                    st_coh(&bbmax[rchild], mx4);
+                    st_coh(&area[rchild], rar);
                     // the reference stores costs[partition] for the right child too (TRBVH.cu:309)
-                    st_coh(&nsah[child], cost[P]);
-                    st_coh(&span[child], ssp[Q]);
-                    stk[slot] = child;
+                    st_coh(&nsah[rchild], cost[P]);
+                    st_coh(&span[rchild], ss);
+                    stk[slot] = rchild;
                     stk[8 + slot] = Q;
                 } else {
-                    child = h.leaf[__ffs(Q) - 1];
+                    const int ri = __ffs(Q) - 1;
+                    rchild = h.leaf[ri];
+                    rar = h.area[ri];
                 }
-                st_coh(&right[sroot], child);
-                st_coh(&parent[child], sroot);
+                st_coh(&right[sroot], rchild);
+                st_coh(&parent[rchild], sroot);
+                st_coh(&rec[sroot], make_int4(lchild, rchild, __float_as_int(lar),
+                                              __float_as_int(rar)));
             }
             next += nl + nr;
             sp = nl + nr;
@@ -639,20 +710,36 @@ struct PassShift<256> {
     static constexpr int value = 8;
 };
 
+// A node's count and SAH travel up with the climb: the first of its two
+// child threads to arrive parks {count, SAH} of its child in the node's
+// 64-bit slot (an LDS slot for block-local nodes, a global slot otherwise) and
+// stops; the second takes them with the same exchange, so it needs no further
+// load to finish the node.  Slots start empty (all ones: no count is
+// 0xFFFFFFFF).
+constexpr uint64_t kSlotEmpty = ~0ull;
+// Roots of at least this many triangles form their treelets breadth-first
+// (measured on the 2M-triangle build: 2% off the passes at 16384, less at
+// 2048, a loss at 512 where the extra loads meet the full-occupancy phase).
+constexpr int kBfsMinTris = 16384;
+__device__ __forceinline__ uint64_t pack_count_sah(int count, float sah)
+{
+    return ((uint64_t)(uint32_t)count << 32) | __float_as_uint(sah);
+}
+
 template <bool AGG, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int first_leaf, int n,
                                                       int gamma, float ci,
                                                       float ct, int32_t* parent, int32_t* left,
                                                       int32_t* right, float4* bbmin,
-                                                      float4* bbmax, float* area, int32_t* tris,
-                                                      float* nsah, uint32_t* counters,
+                                                      float4* bbmax, float* area,
+                                                      float* nsah, uint64_t* slots,
                                                       int2* span, int4* rec,
                                                       const int32_t* sched, int rounds,
                                                       int half_words)
 {
     constexpr int kShift = PassShift<BLOCK>::value;
     extern __shared__ float smem[];
-    __shared__ uint32_t s_cnt[BLOCK];
+    __shared__ uint64_t s_slot[BLOCK];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int half = lane >> 5;
@@ -661,14 +748,33 @@ __global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int f
 
     const uint32_t gid = blockIdx.x * (uint32_t)BLOCK + (uint32_t)tid;
     const int blk = (int)blockIdx.x;
-    s_cnt[tid] = 0u;
+    s_slot[tid] = kSlotEmpty;
     __syncthreads();
+    // `node` is the node this thread holds (count `ncount`, SAH `nsah_v`);
+    // `cur` its parent, the node the thread climbs to next.  A node's own
+    // links, area, span and parent are unchanged from the start of the
+    // launch until the node is processed (only treelets rooted at it or above
+    // rewrite them), so the next node's fields are loaded one step ahead,
+    // plainly, while the thread arrives at and works on the current one.
     int cur = -1;
+    int node = -1;
+    int ncount = 0;
+    float nsah_v = 0.0f;
+    int2 f_sp = make_int2(0, 0);
+    int f_l = -1, f_up = -1;
+    float f_ar = 0.0f;
     if (gid < n_leaves) {
-        const int leaf = first_leaf + (int)gid;
-        st_coh(&tris[leaf], 1);
-        st_coh(&nsah[leaf], ct * area[leaf]);
-        cur = parent[leaf];
+        node = first_leaf + (int)gid;
+        ncount = 1;
+        nsah_v = ct * area[node];
+        st_coh(&nsah[node], nsah_v);
+        cur = parent[node];
+        if (cur >= 0) {
+            f_sp = span[cur];
+            f_l = left[cur];
+            f_ar = area[cur];
+            f_up = parent[cur];
+        }
     }
     // A thread whose node is a treelet root may hold it back for one step
     // (`held`): its arrival, count, SAH and locality are kept, and it joins
@@ -687,39 +793,39 @@ __global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int f
         }
         if (cur >= 0 && !held) {
             // Nodes whose leaves all belong to this block meet in LDS; the rest
-            // meet in global memory (cross-XCD hand-over, see device.hpp).  A
-            // node's own links, area, span and parent are unchanged from the
-            // start of the launch until the node is processed (only treelets
-            // rooted at it or above rewrite them), so they are read plainly and
-            // before the arrival: their latency overlaps the arrival's wait for
-            // this thread's earlier stores.
-            const int2 sp = span[cur];
-            const int l = left[cur];
-            const int r = right[cur];
-            const float ar = area[cur];
-            up = parent[cur];
+            // meet in global memory (cross-XCD hand-over, see device.hpp).
+            const int2 sp = f_sp;
+            const int l = f_l;
+            const float ar = f_ar;
+            up = f_up;
             local = block_local<kShift>(sp, cur, blk);
-            const uint32_t old =
-                local ? arrive_local(&s_cnt[cur & (BLOCK - 1)])
-                      : arrive_crossing(&counters[cur]);
-            if (old == 0u) {
+            const uint64_t mine = pack_count_sah(ncount, nsah_v);
+            // the next node's fields, loaded while this arrival waits for the
+            // thread's earlier stores to complete
+            if (up >= 0) {
+                f_sp = span[up];
+                f_l = left[up];
+                f_ar = area[up];
+                f_up = parent[up];
+            }
+            uint64_t old;
+            if (local) {
+                wait_memory();
+                old = __hip_atomic_exchange(&s_slot[cur & (BLOCK - 1)], mine, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                old = exchange_crossing(&slots[cur], mine);
+            }
+            if (old == kSlotEmpty) {
                 cur = -1;
             } else {
-                int tl, tr;
-                float sl, sr;
-                if (local) {
-                    tl = ld<false>(&tris[l]);
-                    sl = ld<false>(&nsah[l]);
-                    tr = ld<false>(&tris[r]);
-                    sr = ld<false>(&nsah[r]);
-                } else {
-                    tl = ld<true>(&tris[l]);
-                    sl = ld<true>(&nsah[l]);
-                    tr = ld<true>(&tris[r]);
-                    sr = ld<true>(&nsah[r]);
-                }
-                tc = tl + tr;
-                st_coh(&tris[cur], tc);
+                const int oc = (int)(uint32_t)(old >> 32);
+                const float os = __uint_as_float((uint32_t)old);
+                // SAH of the node: ci * area + SAH(left) + SAH(right), in this order
+                const bool is_left = l == node;
+                const float sl = is_left ? nsah_v : os;
+                const float sr = is_left ? os : nsah_v;
+                tc = ncount + oc;
                 my_sah = ci * ar + sl + sr;
                 st_coh(&nsah[cur], my_sah);
             }
@@ -761,45 +867,67 @@ __global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int f
             // are also correct for a block-local treelet), so the wave runs
             // one flavour instead of both in turn.
             const bool wave_local = __ballot(my_r >= 0 && !root_local) == 0ull;
+            // Large subtrees are few and their treelets sit on the pass's
+            // longest chain (towards the root), where latency, not issue
+            // rate, decides: they form breadth-first.
+            const bool wave_big = __ballot(my_r >= 0 && root_tris >= kBfsMinTris) != 0ull;
+            // the root's SAH after the treelet (what its parent adds up)
+            float new_sah = root_sah;
             if (my_r >= 0) {
                 if (AGG) {
-                    if (wave_local)
-                        atrbvh_treelet<false>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
-                                              right, bbmin, bbmax, area, nsah, span, rec);
+                    if (wave_big)
+                        new_sah = atrbvh_treelet<true, true>(lds, hl, root, root_sah, n,
+                                                                   first_leaf, ci, parent, left,
+                                                                   right, bbmin, bbmax, area,
+                                                                   nsah, span, rec);
+                    else if (wave_local)
+                        new_sah = atrbvh_treelet<false, false>(lds, hl, root, root_sah, n,
+                                                                     first_leaf, ci, parent, left,
+                                                                     right, bbmin, bbmax, area,
+                                                                     nsah, span, rec);
                     else
-                        atrbvh_treelet<true>(lds, hl, root, root_sah, n, first_leaf, ci, parent, left,
-                                             right, bbmin, bbmax, area, nsah, span, rec);
+                        new_sah = atrbvh_treelet<true, false>(lds, hl, root, root_sah, n,
+                                                                    first_leaf, ci, parent, left,
+                                                                    right, bbmin, bbmax, area,
+                                                                    nsah, span, rec);
                 } else {
-                    if (wave_local)
-                        trbvh_treelet<false>(lds, hl, root, root_sah, root_tris, first_leaf, ci, ct, sched,
-                                             rounds, parent, left, right, bbmin, bbmax, area,
-                                             nsah, span);
+                    if (wave_big)
+                        trbvh_treelet<true, true>(lds, hl, root, root_sah, root_tris, first_leaf,
+                                                  ci, ct, sched, rounds, parent, left, right,
+                                                  bbmin, bbmax, area, nsah, span, rec);
+                    else if (wave_local)
+                        trbvh_treelet<false, false>(lds, hl, root, root_sah, root_tris,
+                                                    first_leaf, ci, ct, sched, rounds, parent,
+                                                    left, right, bbmin, bbmax, area, nsah, span,
+                                                    rec);
                     else
-                        trbvh_treelet<true>(lds, hl, root, root_sah, root_tris, first_leaf, ci, ct, sched,
-                                            rounds, parent, left, right, bbmin, bbmax, area,
-                                            nsah, span);
+                        trbvh_treelet<true, false>(lds, hl, root, root_sah, root_tris, first_leaf,
+                                                   ci, ct, sched, rounds, parent, left, right,
+                                                   bbmin, bbmax, area, nsah, span, rec);
                 }
             }
+            const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(new_sah), 0));
+            const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(new_sah), 32));
+            if (lane == r0) my_sah = s0;
+            if (lane == r1) my_sah = s1;
         }
-        if (cur >= 0 && !held) cur = up;
+        if (cur >= 0 && !held) {
+            node = cur;
+            ncount = tc;
+            nsah_v = my_sah;
+            cur = up;
+        }
     }
 }
 
-__global__ void k_zero_u32(uint32_t* p, uint64_t count)
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count)
 {
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < count; i += 256ull * gridDim.x)
-        p[i] = 0u;
+        p[i] = v;
 }
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-atr_status zero_counters(atr_context* ctx, uint32_t* counters, uint32_t count)
-{
-    return launch(ctx, "zero_counters", count * 4.0, [&] {
-        k_zero_u32<<<std::min<uint32_t>(std::max(div_up(count, 256), 1u), 2048u), 256, 0,
-                     ctx->stream>>>(counters, count);
-    });
-}
 
 // Node records {left, right, area(left), area(right)} of every internal node
 // from the SoA arrays.  Internal slots whose links are not node indices (the
@@ -834,54 +962,62 @@ atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec)
 
 template <bool AGG, int BLOCK>
 static void launch_pass(atr_context* ctx, atr_tree* t, int n, int gamma, float ci, float ct,
-                        int32_t* tris, float* nsah, uint32_t* counters, int4* rec,
-                        const int32_t* sched, int rounds, int half_words)
+                        float* nsah, uint64_t* slots, int4* rec, const int32_t* sched,
+                        int rounds, int half_words)
 {
     const uint32_t N = t->n;
     const size_t lds = (size_t)half_words * (BLOCK / 32) * sizeof(float);
     k_treelet_pass<AGG, BLOCK><<<div_up(N, BLOCK), BLOCK, lds, ctx->stream>>>(
         N, (int)t->first_leaf, n, gamma, ci, ct, t->parent, t->left, t->right, t->bbmin,
-        t->bbmax, t->area, tris, nsah, counters, t->span, rec, sched, rounds, half_words);
+        t->bbmax, t->area, nsah, slots, t->span, rec, sched, rounds, half_words);
 }
 
 template <bool AGG>
 static atr_status launch_pass_block(atr_context* ctx, const char* name, int block, atr_tree* t,
-                                    int n, int gamma, float ci, float ct, int32_t* tris,
-                                    float* nsah, uint32_t* counters, int4* rec,
-                                    const int32_t* sched, int rounds, int half_words)
+                                    int n, int gamma, float ci, float ct, float* nsah,
+                                    uint64_t* slots, int4* rec, const int32_t* sched, int rounds,
+                                    int half_words)
 {
     if (block != 64 && block != 128 && block != 256)
         return fail(ATR_ERR_INVALID_ARGUMENT, "treelet pass block must be 64, 128 or 256");
     return launch(ctx, name, 0.0, [&] {
         if (block == 64)
-            launch_pass<AGG, 64>(ctx, t, n, gamma, ci, ct, tris, nsah, counters, rec, sched,
-                                 rounds, half_words);
+            launch_pass<AGG, 64>(ctx, t, n, gamma, ci, ct, nsah, slots, rec, sched, rounds,
+                                 half_words);
         else if (block == 128)
-            launch_pass<AGG, 128>(ctx, t, n, gamma, ci, ct, tris, nsah, counters, rec, sched,
-                                  rounds, half_words);
+            launch_pass<AGG, 128>(ctx, t, n, gamma, ci, ct, nsah, slots, rec, sched, rounds,
+                                  half_words);
         else
-            launch_pass<AGG, 256>(ctx, t, n, gamma, ci, ct, tris, nsah, counters, rec, sched,
-                                  rounds, half_words);
+            launch_pass<AGG, 256>(ctx, t, n, gamma, ci, ct, nsah, slots, rec, sched, rounds,
+                                  half_words);
+    });
+}
+
+// Every arrival slot empty (all ones) before a pass.
+atr_status clear_slots(atr_context* ctx, uint64_t* slots, uint32_t count)
+{
+    return launch(ctx, "clear_slots", count * 8.0, [&] {
+        k_fill_u32<<<std::min<uint32_t>(std::max(div_up(2ull * count, 256), 1u), 2048u), 256, 0,
+                     ctx->stream>>>((uint32_t*)slots, 0xFFFFFFFFu, 2ull * count);
     });
 }
 
 atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma, float ci,
-                           float ct, int32_t* tris, float* nsah, uint32_t* counters, int4* rec,
-                           int block)
+                           float ct, float* nsah, uint64_t* slots, int4* rec, int block)
 {
-    ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
+    ATR_TRY(clear_slots(ctx, slots, t->first_leaf));
     const int half_words = kAggFixedWords + treelet * (treelet - 1) / 2;
-    return launch_pass_block<true>(ctx, "atrbvh_pass", block, t, treelet, gamma, ci, ct, tris,
-                                   nsah, counters, rec, nullptr, 0, half_words);
+    return launch_pass_block<true>(ctx, "atrbvh_pass", block, t, treelet, gamma, ci, ct, nsah,
+                                   slots, rec, nullptr, 0, half_words);
 }
 
 atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
-                          int32_t* tris, float* nsah, uint32_t* counters,
-                          const int32_t* d_schedule, int rounds, int block)
+                          float* nsah, uint64_t* slots, int4* rec, const int32_t* d_schedule,
+                          int rounds, int block)
 {
-    ATR_TRY(zero_counters(ctx, counters, t->first_leaf));
-    return launch_pass_block<false>(ctx, "trbvh_pass", block, t, kTrN, gamma, ci, ct, tris, nsah,
-                                    counters, nullptr, d_schedule, rounds, kTrWords);
+    ATR_TRY(clear_slots(ctx, slots, t->first_leaf));
+    return launch_pass_block<false>(ctx, "trbvh_pass", block, t, kTrN, gamma, ci, ct, nsah,
+                                    slots, rec, d_schedule, rounds, kTrWords);
 }
 
 }  // namespace atr
