@@ -114,6 +114,7 @@ def lib():
         L.atr_collapsed_download.argtypes = [_vp, ctypes.POINTER(CollapsedView)]
         L.atr_collapsed_destroy.argtypes = [_vp]
         L.atr_trace.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]
+        L.atr_collapsed_layout.argtypes = [_vp]
         L.atr_rays_pinhole.argtypes = [_vp, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32,
                                        ctypes.c_uint32, ctypes.c_float, _vp]
         L.atr_rays_diffuse.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
@@ -544,6 +545,10 @@ class Collapsed:
         check(lib().atr_collapsed_download(self.h, ctypes.byref(v)))
         out["used"] = v.used
         return out
+
+    def layout(self):
+        """Builds the traversal layout now (trace builds it on first use)."""
+        check(lib().atr_collapsed_layout(self.h))
 
     def trace(self, d_rays: DeviceBuffer, n: int, d_results: DeviceBuffer, any_hit=False):
         check(lib().atr_trace(self.h, d_rays.ptr, n, 1 if any_hit else 0, d_results.ptr))

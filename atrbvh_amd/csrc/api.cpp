@@ -864,9 +864,9 @@ atr_status atr_collapse(atr_tree* t, float ci, float ct, atr_collapsed** out, fl
     const float4* verts = t->scene ? t->scene->verts : nullptr;
     atr_collapsed* c = nullptr;
     ATR_TRY(collapsed_alloc(ctx, t->n, &c));
-    c->has_layout = verts != nullptr;
+    c->has_layout = false;
     c->verts = verts;
-    atr_status s = run_collapse(ctx, t, c, ci, ct, verts);
+    atr_status s = run_collapse(ctx, t, c, ci, ct);
     if (s == ATR_OK) {
         CollapseSummary* dst = (CollapseSummary*)ctx->h_pinned + c->pinned_slot;
         hipError_t e = hipMemcpyAsync(dst, c->d_summary, sizeof(CollapseSummary),
@@ -925,12 +925,22 @@ atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, i
 {
     if (!c || (num_rays && (!d_rays || !d_results)))
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (!c->has_layout)
-        return fail(ATR_ERR_INVALID_ARGUMENT,
-                    "collapsed tree has no traversal layout: its tree was not bound to a scene");
+    ATR_TRY(atr_collapsed_layout(c));
     ATR_TRY(collapsed_summary(c));
     return run_trace(c->ctx, c, (const float4*)d_rays, num_rays, any_hit, d_results,
                      c->h_summary.height);
+}
+
+atr_status atr_collapsed_layout(atr_collapsed* c)
+{
+    if (!c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (c->has_layout) return ATR_OK;
+    if (!c->verts)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "collapsed tree has no traversal layout: its tree was not bound to a scene");
+    ATR_TRY(run_layout(c->ctx, c, c->verts));
+    c->has_layout = true;
+    return ATR_OK;
 }
 
 atr_status atr_collapsed_compact(atr_collapsed* c, int node_offset_size_div,

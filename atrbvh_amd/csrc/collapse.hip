@@ -207,9 +207,8 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
 // made only for trees built from primitive boxes (`dedup`).
 __device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off, bool dedup,
                           const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                          const int32_t* __restrict__ data, const float4* __restrict__ v,
-                          int32_t* c_data, int32_t* c_count, int32_t* tri_index, float4* ttris,
-                          int32_t* ttri_id)
+                          const int32_t* __restrict__ data, int32_t* c_data, int32_t* c_count,
+                          int32_t* tri_index)
 {
     int32_t* buf = tri_index + off;
     buf[cnt - 1] = left[child];
@@ -224,20 +223,7 @@ __device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off, bool ded
             bool fresh = true;
             if (dedup)
                 for (int q = 0; q < pos && fresh; ++q) fresh = buf[q] != tri;
-            if (fresh) {
-                buf[pos] = tri;
-                if (v) {
-                    const uint32_t slot = N + off + (uint32_t)pos;
-                    const float4 a = v[3ull * tri];
-                    const float4 b = v[3ull * tri + 1];
-                    const float4 c = v[3ull * tri + 2];
-                    ttris[3ull * slot] = a;
-                    ttris[3ull * slot + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
-                    ttris[3ull * slot + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
-                    ttri_id[slot] = tri;
-                }
-                ++pos;
-            }
+            if (fresh) buf[pos++] = tri;
         } else {
             const int l = left[idx];
             const int r = right[idx];
@@ -245,17 +231,15 @@ __device__ void emit_list(uint32_t N, int child, int cnt, uint32_t off, bool ded
             buf[cnt - 1 - ss++] = r;
         }
     }
-    // the list's last triangle carries the end mark
-    if (v) ttri_id[N + off + (uint32_t)pos - 1] |= (int32_t)0x80000000u;
     c_data[child] = (int32_t)off;
     c_count[child] = pos;
 }
 
 __global__ __launch_bounds__(256) void k_collapse_emit(
     uint32_t N, bool dedup, const uint32_t* __restrict__ emit_count,
-    const uint32_t* __restrict__ emit_off, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-    const int32_t* __restrict__ data, const float4* __restrict__ v, int32_t* c_data,
-    int32_t* c_count, int32_t* tri_index, float4* ttris, int32_t* ttri_id)
+    const uint32_t* __restrict__ emit_off, const int32_t* __restrict__ left,
+    const int32_t* __restrict__ right, const int32_t* __restrict__ data, int32_t* c_data,
+    int32_t* c_count, int32_t* tri_index)
 {
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p + 1 >= N) return;
@@ -266,13 +250,10 @@ __global__ __launch_bounds__(256) void k_collapse_emit(
     const int cl = c_count[l];
     const int cr = c_count[r];
     if (cl > 1) {
-        emit_list(N, l, cl, off, dedup, left, right, data, v, c_data, c_count, tri_index, ttris,
-                  ttri_id);
+        emit_list(N, l, cl, off, dedup, left, right, data, c_data, c_count, tri_index);
         off += (uint32_t)cl;
     }
-    if (cr > 1)
-        emit_list(N, r, cr, off, dedup, left, right, data, v, c_data, c_count, tri_index, ttris,
-                  ttri_id);
+    if (cr > 1) emit_list(N, r, cr, off, dedup, left, right, data, c_data, c_count, tri_index);
 }
 
 // Single-triangle slots (one per tree leaf, in leaf order).
@@ -290,6 +271,33 @@ __global__ __launch_bounds__(256) void k_leaf_slots(uint32_t N, const int32_t* _
     ttris[3ull * i + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
     ttris[3ull * i + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
     ttri_id[i] = (int32_t)((uint32_t)tri | 0x80000000u);
+}
+
+// The slots of every collapsed list (one thread per internal node that holds
+// a list: count triangles from tri_index at its data offset), end mark on
+// the list's last triangle.
+__global__ __launch_bounds__(256) void k_list_slots(uint32_t N, const int32_t* __restrict__ c_data,
+                                                    const int32_t* __restrict__ c_count,
+                                                    const int32_t* __restrict__ tri_index,
+                                                    const float4* __restrict__ v, float4* ttris,
+                                                    int32_t* ttri_id)
+{
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p + 1 >= N) return;
+    const int off = c_data[p];
+    if (off < 0) return;
+    const int cnt = c_count[p];
+    for (int j = 0; j < cnt; ++j) {
+        const uint32_t slot = N + (uint32_t)off + (uint32_t)j;
+        const int tri = tri_index[off + j];
+        const float4 a = v[3ull * tri];
+        const float4 b = v[3ull * tri + 1];
+        const float4 c = v[3ull * tri + 2];
+        ttris[3ull * slot] = a;
+        ttris[3ull * slot + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+        ttris[3ull * slot + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+        ttri_id[slot] = j == cnt - 1 ? (int32_t)((uint32_t)tri | 0x80000000u) : tri;
+    }
 }
 
 // One 64-byte traversal record per internal node.  Records of nodes inside
@@ -347,8 +355,7 @@ atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32
     return ATR_OK;
 }
 
-atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct,
-                        const float4* verts)
+atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct)
 {
     const uint32_t N = t->n;
     const size_t nodes = 2ull * N - 1;
@@ -373,29 +380,35 @@ atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float c
             emit_count, counters, t->span, c->d_summary);
     }));
     ATR_TRY(run_scan_u32(ctx, emit_count, N - 1, emit_off, &c->d_summary->used));
-    if (verts) {
-        ATR_TRY(launch(ctx, "leaf_slots", N * 68.0, [&] {
-            k_leaf_slots<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->data, verts, c->ttris,
-                                                                   c->ttri_id);
-        }));
-    }
     ATR_TRY(launch(ctx, "collapse_emit", 0.0, [&] {
         k_collapse_emit<<<div_up(N, 256), 256, 0, ctx->stream>>>(
-            N, t->box_leaves, emit_count, emit_off, t->left, t->right, t->data, verts, c->data,
-            c->count,
-            c->tri_index, c->ttris, c->ttri_id);
+            N, t->box_leaves, emit_count, emit_off, t->left, t->right, t->data, c->data,
+            c->count, c->tri_index);
     }));
-    if (verts) {
-        ATR_TRY(launch(ctx, "layout", (double)(N - 1) * 64.0 * 2.0, [&] {
-            k_layout<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->left, c->right, c->data,
-                                                              c->bbmin, c->bbmax, c->tnodes);
-        }));
-    }
     dfree(ctx, state);
     dfree(ctx, emit_count);
     dfree(ctx, emit_off);
     dfree(ctx, counters);
     return ATR_OK;
+}
+
+// The traversal layout of a collapsed tree: single-triangle slots, list
+// slots and node records (built on the first trace, or on request).
+atr_status run_layout(atr_context* ctx, atr_collapsed* c, const float4* verts)
+{
+    const uint32_t N = c->n;
+    ATR_TRY(launch(ctx, "leaf_slots", N * 68.0, [&] {
+        k_leaf_slots<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->data, verts, c->ttris,
+                                                               c->ttri_id);
+    }));
+    ATR_TRY(launch(ctx, "list_slots", 0.0, [&] {
+        k_list_slots<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->data, c->count, c->tri_index,
+                                                               verts, c->ttris, c->ttri_id);
+    }));
+    return launch(ctx, "layout", (double)(N - 1) * 64.0 * 2.0, [&] {
+        k_layout<<<div_up(N, 256), 256, 0, ctx->stream>>>(N, c->left, c->right, c->data,
+                                                          c->bbmin, c->bbmax, c->tnodes);
+    });
 }
 
 }  // namespace atr

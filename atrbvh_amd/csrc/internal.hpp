@@ -312,11 +312,11 @@ atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, fl
                           float* nsah, uint64_t* slots, int4* rec, const int32_t* d_schedule,
                           int rounds, int block = 128);
 
-// Collapse + traversal layout (collapse.hip)
-// verts may be NULL: then only the reference collapsed tree is produced
-// (no traversal layout).
-atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct,
-                        const float4* verts);
+// Collapse + traversal layout (collapse.hip): the reference's collapsed tree,
+// and, separately, the traversal layout built from it and the scene's
+// vertices (on the first trace or on request).
+atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct);
+atr_status run_layout(atr_context* ctx, atr_collapsed* c, const float4* verts);
 atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32_t* out,
                         uint32_t* total);
 

@@ -5,8 +5,10 @@ Workload (the largest single-GPU configuration of the workload set):
 in [0,1]^3, per-cluster sigma log-uniform in [0.005, 0.1], equilateral
 triangles with log-normal edge length, median 0.002, sigma 0.7, random
 rotation).  One step = LBVH with 63-bit Morton keys -> ATRBVH (treelet 9,
-2 iterations) -> SAH leaf collapse (ci 1.2, ct 1), with the scene resident in
-HBM.  value = triangles / step time (Mtris/s).
+2 iterations) -> SAH leaf collapse (ci 1.2, ct 1) to the reference's
+collapsed tree, with the scene resident in HBM.  value = triangles / step
+time (Mtris/s).  The traversal layout built from the collapsed tree (the
+reference builds its CudaBVH after the build, on the host) is timed apart.
 
 Also reported: SAH after each stage, the TRBVH (treelet 7, 3 iterations)
 build, traversal Mrays/s for 4,194,304 primary rays (2048x2048 pinhole at
@@ -579,6 +581,13 @@ def main():
     sah_atrbvh = tree.SAH()
     collapsed, sah_collapsed = ab.BVHTreeCollapser().Collapse(tree)
     stats = collapsed.stats()
+    # the traversal layout (node records and triangle slots), built from the
+    # collapsed tree after the build, as CudaBVH is after the reference's
+    ctx.sync()
+    lt0 = time.perf_counter()
+    collapsed.layout()
+    ctx.sync()
+    tlayout_ms = (time.perf_counter() - lt0) * 1000.0
 
     # ---------------------------------------------------------------- TRBVH build
     ctx.sync()
@@ -748,6 +757,9 @@ def main():
                 "atrbvh_collapsed": round(sah_collapsed, 4), "trbvh": round(sah_trbvh, 4),
                 "trbvh_collapsed": round(sah_trbvh_collapsed, 4)},
         "tree_height": stats["height"],
+        "traversal_layout": {"ms": round(tlayout_ms, 3),
+                             "build_plus_layout_Mtris_s": round(
+                                 N_TRIS / ((ms_per_step + tlayout_ms) * 1e-3) / 1e6, 2)},
         "trbvh_build": {"ms": round(trbvh_ms, 3),
                         "Mtris_s": round(N_TRIS / (trbvh_ms * 1e-3) / 1e6, 2)},
         "trace": trace,

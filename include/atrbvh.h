@@ -161,9 +161,18 @@ atr_status atr_atrbvh_optimize(atr_tree* tree, int treelet_size, int iterations,
  * BVHTreeCollapser.cpp:21-93).  *sah (nullable) = cost[root]/area[root]; when
  * sah is non-NULL the call blocks until it is known (as the reference does),
  * otherwise it is asynchronous and atr_collapsed_stats gives the SAH later.
- * When the tree is bound to a scene the collapsed tree also carries the
- * traversal layout atr_trace needs. */
+ * The result is the reference's collapsed tree; when the tree is bound to a
+ * scene, the collapsed tree keeps that scene's vertices for its traversal
+ * layout (atr_collapsed_layout). */
 atr_status atr_collapse(atr_tree* tree, float ci, float ct, atr_collapsed** out, float* sah);
+
+/* The traversal layout atr_trace walks (64-byte node records and (v0, e1,
+ * e2) triangle slots, built from the collapsed tree and its scene's vertices
+ * on the GPU; the step CudaBVH takes on the host after the reference's
+ * collapse, gpu-ray-traversal/src/rt/cuda/CudaBVH.cpp:469-574).  atr_trace
+ * builds it on its first call; this builds it now (stream-ordered).  Fails
+ * when the tree was not bound to a scene. */
+atr_status atr_collapsed_layout(atr_collapsed* c);
 
 /* SoABVHTree::SAH (BVHRT-Core/src/SoABVHTree.cpp:50-74), same summation order;
  * blocking (downloads the area array). */
@@ -190,8 +199,8 @@ atr_status atr_tree_destroy(atr_tree* tree);
 /* Tree from a host view (HostToDeviceTree, BVHTreeInstanceManager.cpp:56-66). */
 atr_status atr_tree_create_from_host(atr_context* ctx, const atr_tree_view* host,
                                      atr_tree** out);
-/* Trees remember the scene they were built from; collapse uses its vertices
- * to emit the traversal layout.  Uploaded trees are bound here.  The scene
+/* Trees remember the scene they were built from; their collapsed trees use
+ * its vertices for the traversal layout.  Uploaded trees are bound here.  The scene
  * must outlive the trees bound to it. */
 atr_status atr_tree_bind_scene(atr_tree* tree, atr_scene* scene);
 
