@@ -13,32 +13,18 @@ namespace atr {
 
 namespace {
 
-// Orderable unsigned image of a float: unsigned order equals float order.
-__device__ __forceinline__ uint32_t f2o(float f)
-{
-    const uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float o2f(uint32_t o)
-{
-    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
-}
-
-__global__ void k_ray_box_init(uint32_t* acc)
-{
-    const int t = threadIdx.x;
-    if (t < 3) acc[t] = f2o(FLT_MAX);
-    else if (t < 6) acc[t] = f2o(-FLT_MAX);
-}
-
 // findAABBKernel (RayBufferKernels.cu:64-135): the box of every ray's origin
 // and end point o + d * tmax.  An end point that is not finite (tmax = inf,
 // the convention of closest-hit rays without a far plane) is left out; the
-// reference's rays always carry a finite tmax.  min / max are exact, so the
-// order of the reduction does not matter.
+// reference's rays always carry a finite tmax.  Each block reduces its rays
+// (waves by cross-lane min / max, then the block's waves through LDS) to one
+// partial box; a single block then reduces the partials.  min / max are
+// exact, so the order of the reduction does not matter.
+constexpr int kBoxBlocks = 1024;
 __global__ __launch_bounds__(256) void k_ray_box(const float4* __restrict__ rays, uint32_t r,
-                                                 uint32_t* acc)
+                                                 float* __restrict__ partial)
 {
+    __shared__ float s[4][6];
     float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
     float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < r; i += 256u * gridDim.x) {
@@ -68,17 +54,55 @@ __global__ __launch_bounds__(256) void k_ray_box(const float4* __restrict__ rays
             mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
         }
     }
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0u) {
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&acc[a], f2o(mn[a]));
-            atomicMax(&acc[3 + a], f2o(mx[a]));
+            s[wave][a] = mn[a];
+            s[wave][3 + a] = mx[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float v = s[0][a];
+        for (int w = 1; w < 4; ++w) v = a < 3 ? fminf(v, s[w][a]) : fmaxf(v, s[w][a]);
+        partial[6ull * blockIdx.x + a] = v;
     }
 }
 
-__global__ void k_ray_box_final(const uint32_t* acc, float* box)
+// The partial boxes of `blocks` blocks to the rays' box (one block).
+__global__ __launch_bounds__(256) void k_ray_box_final(const float* __restrict__ partial,
+                                                       uint32_t blocks, float* __restrict__ box)
 {
-    if (threadIdx.x < 6) box[threadIdx.x] = o2f(acc[threadIdx.x]);
+    __shared__ float s[4][6];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256u) {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], partial[6ull * b + a]);
+            mx[a] = fmaxf(mx[a], partial[6ull * b + 3 + a]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, 64));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, 64));
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0u) {
+        for (int a = 0; a < 3; ++a) {
+            s[wave][a] = mn[a];
+            s[wave][3 + a] = mx[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float v = s[0][a];
+        for (int w = 1; w < 4; ++w) v = a < 3 ? fminf(v, s[w][a]) : fmaxf(v, s[w][a]);
+        box[a] = v;
+    }
 }
 
 // float -> uint32 with the saturation of the GPU conversion the reference
@@ -183,21 +207,20 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 atr_status run_ray_keys(atr_context* ctx, const float4* rays, uint32_t r, uint64_t* keys,
                         float* d_box)
 {
-    uint32_t* acc = nullptr;
-    ATR_TRY(dalloc_t(ctx, 8, &acc));
-    ATR_TRY(launch(ctx, "ray_box_init", 0, [&] { k_ray_box_init<<<1, 64, 0, ctx->stream>>>(acc); }));
-    const uint32_t grid = std::max(std::min<uint32_t>(div_up(r, 256 * 8), 2048u), 1u);
+    float* partial = nullptr;
+    const uint32_t blocks = std::max(std::min<uint32_t>(div_up(r, 256 * 8), kBoxBlocks), 1u);
+    ATR_TRY(dalloc_t(ctx, 6ull * blocks, &partial));
     ATR_TRY(launch(ctx, "ray_box", (double)r * 32.0, [&] {
-        k_ray_box<<<grid, 256, 0, ctx->stream>>>(rays, r, acc);
+        k_ray_box<<<blocks, 256, 0, ctx->stream>>>(rays, r, partial);
     }));
-    ATR_TRY(launch(ctx, "ray_box_final", 0, [&] {
-        k_ray_box_final<<<1, 64, 0, ctx->stream>>>(acc, d_box);
+    ATR_TRY(launch(ctx, "ray_box_final", blocks * 24.0, [&] {
+        k_ray_box_final<<<1, 256, 0, ctx->stream>>>(partial, blocks, d_box);
     }));
     atr_status s = launch(ctx, "ray_keys", (double)r * (32.0 + 24.0), [&] {
         k_ray_keys<<<std::max(div_up(r, 256), 1u), 256, 0, ctx->stream>>>(
             rays, r, d_box, keys, keys + r, keys + 2ull * r);
     });
-    dfree(ctx, acc);
+    dfree(ctx, partial);
     return s;
 }
 
