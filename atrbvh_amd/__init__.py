@@ -98,6 +98,15 @@ def lib():
         L.atr_atrbvh_optimize.argtypes = L.atr_trbvh_optimize.argtypes
         L.atr_collapse.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.POINTER(_vp),
                                    _f32p]
+        L.atr_build_tree.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.POINTER(_vp)]
+        L.atr_build_tree_boxes.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_uint32,
+                                           ctypes.c_int, ctypes.POINTER(_vp)]
+        L.atr_trbvh_pass.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_float]
+        L.atr_atrbvh_pass.argtypes = L.atr_trbvh_pass.argtypes
+        L.atr_collapse_costs.argtypes = [_vp, ctypes.c_float, ctypes.c_float, ctypes.POINTER(_vp)]
+        L.atr_collapse_lists.argtypes = [_vp, _vp, _f32p]
         L.atr_collapsed_stats.argtypes = [_vp, _f32p, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_uint32)]
         L.atr_tree_sah.argtypes = [_vp, ctypes.c_float, ctypes.c_float, _f32p]
@@ -722,3 +731,52 @@ class BVHTreeCollapser:
         check(lib().atr_collapse(tree.h, ci, ct, ctypes.byref(h),
                                  ctypes.byref(sah) if want_sah else None))
         return Collapsed(tree.ctx, h.value, tree.n), (sah.value if want_sah else None)
+
+
+# ------------------------------------------------------- stages one at a time
+def DeviceBuildTree(scene: Scene, d_sorted_keys: DeviceBuffer, d_sort_indices: DeviceBuffer,
+                    n: int, morton_bits: int) -> Tree:
+    """DeviceBuildTree + DeviceCalculateNodeBoundingBoxes (LBVH.h:80-95,137-139)
+    over keys sorted by Context.sort_pairs (uint32 keys for 30 bits, uint64 for 63)."""
+    h = _vp()
+    check(lib().atr_build_tree(scene.h, d_sorted_keys.ptr, d_sort_indices.ptr, n, morton_bits,
+                               ctypes.byref(h)))
+    return Tree(scene.ctx, h.value, n)
+
+
+def DeviceBuildTreeTS(scene: Scene, d_sorted_keys: DeviceBuffer, d_routing: DeviceBuffer,
+                      d_bbmin: DeviceBuffer, d_bbmax: DeviceBuffer, d_tri: DeviceBuffer, m: int,
+                      morton_bits: int) -> Tree:
+    """DeviceBuildTree with routing indices + DeviceCalculateInternalNodeBoundingBoxes
+    (LBVH.h:108-125,153-155) over m primitive boxes."""
+    h = _vp()
+    check(lib().atr_build_tree_boxes(scene.h, d_sorted_keys.ptr, d_routing.ptr, d_bbmin.ptr,
+                                     d_bbmax.ptr, d_tri.ptr, m, morton_bits, ctypes.byref(h)))
+    return Tree(scene.ctx, h.value, m)
+
+
+def DeviceTreeletReestructureOptimizer(tree: Tree, treeletSize: int, gamma: int,
+                                       ci: float = 1.2, ct: float = 1.0):
+    """One TRBVH pass at threshold gamma (TRBVH.h:42-46)."""
+    check(lib().atr_trbvh_pass(tree.h, treeletSize, gamma, ci, ct))
+
+
+def DeviceAgglomerativeTreeletOptimizer(tree: Tree, treeletSize: int, gamma: int,
+                                        ci: float = 1.2, ct: float = 1.0):
+    """One ATRBVH pass at threshold gamma (AgglomerativeTreelet.h:39-75)."""
+    check(lib().atr_atrbvh_pass(tree.h, treeletSize, gamma, ci, ct))
+
+
+def DeviceCalculateCosts(tree: Tree, ci: float = 1.2, ct: float = 1.0) -> Collapsed:
+    """DeviceCopyTree + DeviceCalculateCosts (TreeCollapser.h:19-42): an
+    unfinished collapsed tree that DeviceCollapseTree completes."""
+    h = _vp()
+    check(lib().atr_collapse_costs(tree.h, ci, ct, ctypes.byref(h)))
+    return Collapsed(tree.ctx, h.value, tree.n)
+
+
+def DeviceCollapseTree(tree: Tree, collapsed: Collapsed, want_sah: bool = True):
+    """DeviceCollapseTree (TreeCollapser.h:60-62); returns the SAH (or None)."""
+    sah = ctypes.c_float()
+    check(lib().atr_collapse_lists(tree.h, collapsed.h, ctypes.byref(sah) if want_sah else None))
+    return sah.value if want_sah else None

@@ -546,6 +546,58 @@ atr_status atr_lbvh_build_boxes(atr_scene* scene, const float* d_bbmin, const fl
     return ATR_OK;
 }
 
+// The radix tree + boxes stage over caller-sorted keys (either leaf source).
+static atr_status build_stage(atr_scene* scene, const void* keys, const uint32_t* idx,
+                              uint32_t n, int morton_bits, const float4* bmn, const float4* bmx,
+                              const int32_t* tri, atr_tree** out)
+{
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
+    if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a tree needs at least 2 leaves");
+    if (n > (1u << 30)) return fail(ATR_ERR_INVALID_ARGUMENT, "too many leaves");
+    atr_context* ctx = scene->ctx;
+    atr_tree* t = nullptr;
+    ATR_TRY(tree_alloc(ctx, n, &t));
+    uint64_t* slots = nullptr;
+    atr_status s = dalloc_t(ctx, n - 1, &slots);
+    if (s == ATR_OK) s = clear_slots(ctx, slots, n - 1);
+    if (s == ATR_OK)
+        s = bmn ? run_build_boxes(ctx, morton_bits, keys, idx, n, bmn, bmx, tri, t, slots)
+                : run_build(ctx, morton_bits, keys, idx, n, scene->verts, t, slots);
+    dfree(ctx, slots);
+    if (s != ATR_OK) {
+        atr_tree_destroy(t);
+        return s;
+    }
+    t->scene = scene;
+    t->box_leaves = bmn != nullptr;
+    *out = t;
+    return ATR_OK;
+}
+
+atr_status atr_build_tree(atr_scene* scene, const void* d_sorted_keys,
+                          const uint32_t* d_sort_indices, uint32_t num_triangles, int morton_bits,
+                          atr_tree** out)
+{
+    if (!scene || !d_sorted_keys || !d_sort_indices || !out)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (num_triangles != scene->n)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "num_triangles differs from the scene's");
+    return build_stage(scene, d_sorted_keys, d_sort_indices, num_triangles, morton_bits, nullptr,
+                       nullptr, nullptr, out);
+}
+
+atr_status atr_build_tree_boxes(atr_scene* scene, const void* d_sorted_keys,
+                                const uint32_t* d_routing, const float* d_bbmin,
+                                const float* d_bbmax, const int32_t* d_tri, uint32_t num_boxes,
+                                int morton_bits, atr_tree** out)
+{
+    if (!scene || !d_sorted_keys || !d_routing || !d_bbmin || !d_bbmax || !d_tri || !out)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    return build_stage(scene, d_sorted_keys, d_routing, num_boxes, morton_bits,
+                       (const float4*)d_bbmin, (const float4*)d_bbmax, d_tri, out);
+}
+
 atr_status atr_tree_bind_scene(atr_tree* t, atr_scene* scene)
 {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
@@ -556,8 +608,9 @@ atr_status atr_tree_bind_scene(atr_tree* t, atr_scene* scene)
     return ATR_OK;
 }
 
-static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int iterations, float ci,
-                                   float ct)
+// `iterations` passes at gamma, 2 gamma, 4 gamma, ...
+static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int gamma, int iterations,
+                                   float ci, float ct)
 {
     atr_context* ctx = t->ctx;
     const uint32_t n = t->n;
@@ -585,7 +638,6 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int itera
             if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "schedule upload failed");
         }
     }
-    int gamma = treelet;
     for (int it = 0; it < iterations && s == ATR_OK; ++it) {
         if (agg)
             s = run_atrbvh_pass(ctx, t, treelet, gamma, ci, ct, nsah, slots, rec);
@@ -606,7 +658,7 @@ atr_status atr_trbvh_optimize(atr_tree* t, int treelet_size, int iterations, flo
     if (treelet_size != 7)
         return fail(ATR_ERR_UNSUPPORTED, "TRBVH supports treelet size 7 only");
     if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
-    return treelet_optimize(t, false, treelet_size, iterations, ci, ct);
+    return treelet_optimize(t, false, treelet_size, treelet_size, iterations, ci, ct);
 }
 
 atr_status atr_atrbvh_optimize(atr_tree* t, int treelet_size, int iterations, float ci, float ct)
@@ -615,7 +667,25 @@ atr_status atr_atrbvh_optimize(atr_tree* t, int treelet_size, int iterations, fl
     if (treelet_size < 3 || treelet_size > 32)
         return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
     if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
-    return treelet_optimize(t, true, treelet_size, iterations, ci, ct);
+    return treelet_optimize(t, true, treelet_size, treelet_size, iterations, ci, ct);
+}
+
+atr_status atr_trbvh_pass(atr_tree* t, int treelet_size, int gamma, float ci, float ct)
+{
+    if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
+    if (treelet_size != 7)
+        return fail(ATR_ERR_UNSUPPORTED, "TRBVH supports treelet size 7 only");
+    if (gamma < treelet_size) return fail(ATR_ERR_INVALID_ARGUMENT, "gamma must be >= treelet_size");
+    return treelet_optimize(t, false, treelet_size, gamma, 1, ci, ct);
+}
+
+atr_status atr_atrbvh_pass(atr_tree* t, int treelet_size, int gamma, float ci, float ct)
+{
+    if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
+    if (treelet_size < 3 || treelet_size > 32)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
+    if (gamma < treelet_size) return fail(ATR_ERR_INVALID_ARGUMENT, "gamma must be >= treelet_size");
+    return treelet_optimize(t, true, treelet_size, gamma, 1, ci, ct);
 }
 
 atr_status atr_tree_num_triangles(const atr_tree* t, uint32_t* n)
@@ -839,6 +909,7 @@ atr_status atr_collapsed_destroy(atr_collapsed* c)
     dfree(ctx, c->cwoop);
     dfree(ctx, c->crows_index);
     dfree(ctx, c->d_summary);
+    dfree(ctx, c->stage_emit_count);
     // The event and pinned slot go back to their pools without waiting: a
     // later user records / copies after this tree's pending work in stream
     // order and waits on its own event.
@@ -848,8 +919,17 @@ atr_status atr_collapsed_destroy(atr_collapsed* c)
     return ATR_OK;
 }
 
+static atr_status collapsed_finished(const atr_collapsed* c)
+{
+    if (c->stage_emit_count)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "collapsed tree is unfinished: atr_collapse_lists has not run on it");
+    return ATR_OK;
+}
+
 static atr_status collapsed_summary(atr_collapsed* c)
 {
+    ATR_TRY(collapsed_finished(c));
     if (c->summary_ready) return ATR_OK;
     ATR_HIP(hipEventSynchronize(c->summary_event));
     c->h_summary = ((const CollapseSummary*)c->ctx->h_pinned)[c->pinned_slot];
@@ -857,33 +937,78 @@ static atr_status collapsed_summary(atr_collapsed* c)
     return ATR_OK;
 }
 
+// Allocates the collapsed tree of `t` (bound to t's scene's vertices for its
+// traversal layout).
+static atr_status collapse_begin(atr_tree* t, atr_collapsed** out)
+{
+    atr_collapsed* c = nullptr;
+    ATR_TRY(collapsed_alloc(t->ctx, t->n, &c));
+    c->has_layout = false;
+    c->verts = t->scene ? t->scene->verts : nullptr;
+    *out = c;
+    return ATR_OK;
+}
+
+// Queues the summary's readback after the list stage; *sah (nullable) blocks.
+static atr_status collapse_finish(atr_collapsed* c, float* sah)
+{
+    atr_context* ctx = c->ctx;
+    CollapseSummary* dst = (CollapseSummary*)ctx->h_pinned + c->pinned_slot;
+    hipError_t e = hipMemcpyAsync(dst, c->d_summary, sizeof(CollapseSummary),
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->summary_event, ctx->stream);
+    if (e != hipSuccess) return fail(ATR_ERR_DEVICE, "summary readback failed");
+    if (sah) {
+        ATR_TRY(collapsed_summary(c));
+        *sah = c->h_summary.cost_root / c->h_summary.area_root;
+    }
+    return ATR_OK;
+}
+
 atr_status atr_collapse(atr_tree* t, float ci, float ct, atr_collapsed** out, float* sah)
 {
     if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
-    atr_context* ctx = t->ctx;
-    const float4* verts = t->scene ? t->scene->verts : nullptr;
     atr_collapsed* c = nullptr;
-    ATR_TRY(collapsed_alloc(ctx, t->n, &c));
-    c->has_layout = false;
-    c->verts = verts;
-    atr_status s = run_collapse(ctx, t, c, ci, ct);
-    if (s == ATR_OK) {
-        CollapseSummary* dst = (CollapseSummary*)ctx->h_pinned + c->pinned_slot;
-        hipError_t e = hipMemcpyAsync(dst, c->d_summary, sizeof(CollapseSummary),
-                                      hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(c->summary_event, ctx->stream);
-        if (e != hipSuccess) s = fail(ATR_ERR_DEVICE, "summary readback failed");
-    }
-    if (s == ATR_OK && sah) {
-        s = collapsed_summary(c);
-        if (s == ATR_OK) *sah = c->h_summary.cost_root / c->h_summary.area_root;
-    }
+    ATR_TRY(collapse_begin(t, &c));
+    atr_status s = run_collapse(t->ctx, t, c, ci, ct);
+    if (s == ATR_OK) s = collapse_finish(c, sah);
     if (s != ATR_OK) {
         atr_collapsed_destroy(c);
         return s;
     }
     *out = c;
     return ATR_OK;
+}
+
+atr_status atr_collapse_costs(atr_tree* t, float ci, float ct, atr_collapsed** out)
+{
+    if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    atr_collapsed* c = nullptr;
+    ATR_TRY(collapse_begin(t, &c));
+    atr_status s = dalloc_t(t->ctx, t->n, &c->stage_emit_count);
+    if (s == ATR_OK) s = run_collapse_costs(t->ctx, t, c, ci, ct, c->stage_emit_count);
+    if (s != ATR_OK) {
+        atr_collapsed_destroy(c);
+        return s;
+    }
+    c->stage_tree = t;
+    *out = c;
+    return ATR_OK;
+}
+
+atr_status atr_collapse_lists(atr_tree* t, atr_collapsed* c, float* sah)
+{
+    if (!t || !c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!c->stage_emit_count)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "collapsed tree does not wait for its lists");
+    if (c->stage_tree != t)
+        return fail(ATR_ERR_INVALID_ARGUMENT,
+                    "tree is not the one atr_collapse_costs computed the costs of");
+    ATR_TRY(run_collapse_lists(c->ctx, t, c, c->stage_emit_count));
+    dfree(c->ctx, c->stage_emit_count);
+    c->stage_emit_count = nullptr;
+    c->stage_tree = nullptr;
+    return collapse_finish(c, sah);
 }
 
 atr_status atr_collapsed_stats(atr_collapsed* c, float* sah, int* height, uint32_t* used)
@@ -934,6 +1059,7 @@ atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, i
 atr_status atr_collapsed_layout(atr_collapsed* c)
 {
     if (!c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    ATR_TRY(collapsed_finished(c));
     if (c->has_layout) return ATR_OK;
     if (!c->verts)
         return fail(ATR_ERR_INVALID_ARGUMENT,
@@ -947,6 +1073,7 @@ atr_status atr_collapsed_compact(atr_collapsed* c, int node_offset_size_div,
                                  atr_compact_layout* out)
 {
     if (!c || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    ATR_TRY(collapsed_finished(c));
     if (node_offset_size_div != 1 && node_offset_size_div != 16)
         return fail(ATR_ERR_INVALID_ARGUMENT, "node_offset_size_div must be 1 or 16");
     if (!c->verts)

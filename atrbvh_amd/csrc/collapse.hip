@@ -355,41 +355,62 @@ atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32
     return ATR_OK;
 }
 
-atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct)
+// The copy and cost stage: emit_count[p] (N - 1 words) receives the number
+// of list triangles internal node p emits (zero for most nodes).
+atr_status run_collapse_costs(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct,
+                              uint32_t* emit_count)
 {
     const uint32_t N = t->n;
     const size_t nodes = 2ull * N - 1;
     int4* state = nullptr;
-    uint32_t* emit_count = nullptr;
-    uint32_t* emit_off = nullptr;
     uint32_t* counters = nullptr;
-    ATR_TRY(dalloc_t(ctx, nodes, &state));
-    ATR_TRY(dalloc_t(ctx, N, &emit_count));
-    ATR_TRY(dalloc_t(ctx, N, &emit_off));
-    ATR_TRY(dalloc_t(ctx, N, &counters));
-    ATR_TRY(launch(ctx, "collapse_zero", N * 8.0, [&] {
-        k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
-            counters, 0u, N);
-        k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
-            emit_count, 0u, N);
-    }));
-    ATR_TRY(launch(ctx, "collapse_cost", (double)N * 2.0 * 100.0, [&] {
-        k_collapse_cost<<<div_up(N, 256), 256, 0, ctx->stream>>>(
-            N, ci, ct, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
-            c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, state,
-            emit_count, counters, t->span, c->d_summary);
-    }));
-    ATR_TRY(run_scan_u32(ctx, emit_count, N - 1, emit_off, &c->d_summary->used));
-    ATR_TRY(launch(ctx, "collapse_emit", 0.0, [&] {
-        k_collapse_emit<<<div_up(N, 256), 256, 0, ctx->stream>>>(
-            N, t->box_leaves, emit_count, emit_off, t->left, t->right, t->data, c->data,
-            c->count, c->tri_index);
-    }));
+    atr_status s = dalloc_t(ctx, nodes, &state);
+    if (s == ATR_OK) s = dalloc_t(ctx, N, &counters);
+    if (s == ATR_OK)
+        s = launch(ctx, "collapse_zero", N * 8.0, [&] {
+            k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
+                counters, 0u, N);
+            k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
+                emit_count, 0u, N);
+        });
+    if (s == ATR_OK)
+        s = launch(ctx, "collapse_cost", (double)N * 2.0 * 100.0, [&] {
+            k_collapse_cost<<<div_up(N, 256), 256, 0, ctx->stream>>>(
+                N, ci, ct, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
+                c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, state,
+                emit_count, counters, t->span, c->d_summary);
+        });
     dfree(ctx, state);
-    dfree(ctx, emit_count);
-    dfree(ctx, emit_off);
     dfree(ctx, counters);
-    return ATR_OK;
+    return s;
+}
+
+// The list stage: offsets by an exclusive scan of emit_count, then the lists.
+atr_status run_collapse_lists(atr_context* ctx, atr_tree* t, atr_collapsed* c,
+                              const uint32_t* emit_count)
+{
+    const uint32_t N = t->n;
+    uint32_t* emit_off = nullptr;
+    ATR_TRY(dalloc_t(ctx, N, &emit_off));
+    atr_status s = run_scan_u32(ctx, emit_count, N - 1, emit_off, &c->d_summary->used);
+    if (s == ATR_OK)
+        s = launch(ctx, "collapse_emit", 0.0, [&] {
+            k_collapse_emit<<<div_up(N, 256), 256, 0, ctx->stream>>>(
+                N, t->box_leaves, emit_count, emit_off, t->left, t->right, t->data, c->data,
+                c->count, c->tri_index);
+        });
+    dfree(ctx, emit_off);
+    return s;
+}
+
+atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct)
+{
+    uint32_t* emit_count = nullptr;
+    ATR_TRY(dalloc_t(ctx, t->n, &emit_count));
+    atr_status s = run_collapse_costs(ctx, t, c, ci, ct, emit_count);
+    if (s == ATR_OK) s = run_collapse_lists(ctx, t, c, emit_count);
+    dfree(ctx, emit_count);
+    return s;
 }
 
 // The traversal layout of a collapsed tree: single-triangle slots, list

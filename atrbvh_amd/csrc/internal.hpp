@@ -151,6 +151,10 @@ struct atr_collapsed {
     int pinned_slot = -1;
     bool summary_ready = false;
     hipEvent_t summary_event = nullptr;
+    // between atr_collapse_costs and atr_collapse_lists: the tree the costs
+    // were computed from and each internal node's list length
+    const atr_tree* stage_tree = nullptr;
+    uint32_t* stage_emit_count = nullptr;
 };
 
 // Many independent meshes built as one forest (batched builds).
@@ -306,6 +310,7 @@ atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma
 // read while forming treelets (first_leaf entries); built before the first
 // pass of every optimisation and kept current by the passes.
 atr_status run_make_records(atr_context* ctx, atr_tree* t, int4* rec);
+atr_status clear_slots(atr_context* ctx, uint64_t* slots, uint32_t count);
 // slots: first_leaf 64-bit arrival slots of the passes (cleared by each pass);
 // nsah: 2n-1 node SAH values.
 atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, float ct,
@@ -316,6 +321,10 @@ atr_status run_trbvh_pass(atr_context* ctx, atr_tree* t, int gamma, float ci, fl
 // and, separately, the traversal layout built from it and the scene's
 // vertices (on the first trace or on request).
 atr_status run_collapse(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct);
+atr_status run_collapse_costs(atr_context* ctx, atr_tree* t, atr_collapsed* c, float ci, float ct,
+                              uint32_t* emit_count);
+atr_status run_collapse_lists(atr_context* ctx, atr_tree* t, atr_collapsed* c,
+                              const uint32_t* emit_count);
 atr_status run_layout(atr_context* ctx, atr_collapsed* c, const float4* verts);
 atr_status run_scan_u32(atr_context* ctx, const uint32_t* in, uint32_t n, uint32_t* out,
                         uint32_t* total);

@@ -174,6 +174,63 @@ atr_status atr_collapse(atr_tree* tree, float ci, float ct, atr_collapsed** out,
  * when the tree was not bound to a scene. */
 atr_status atr_collapsed_layout(atr_collapsed* c);
 
+/* ----------------------------------------------------------- stage calls */
+/* The builders' stages one at a time, as the reference's low-level Device*
+ * functions expose them.  Each queues its kernels on the context's stream and
+ * takes its scratch from the context's caching allocator (the reference's
+ * callers pass scratch in); results are bit-identical to the composite calls
+ * above, which run the same kernels in the same order. */
+
+/* DeviceBuildTree (BVHRT-Core/src/LBVH.h:80-95, buildTreeKernel /
+ * buildTree64Kernel, LBVH.cu:208-316) and DeviceCalculateNodeBoundingBoxes
+ * (LBVH.h:137-139, LBVH.cu:465-570) in one kernel: the radix tree over
+ * num_triangles keys sorted ascending (d_sorted_keys: uint32 for morton_bits
+ * 30, uint64 for 63, as atr_morton_encode + atr_sort_pairs give them), leaf i
+ * holding triangle d_sort_indices[i] with that triangle's box, and every
+ * internal node's box and area.  The tree is bound to `scene`, whose triangle
+ * count must be num_triangles. */
+atr_status atr_build_tree(atr_scene* scene, const void* d_sorted_keys,
+                          const uint32_t* d_sort_indices, uint32_t num_triangles,
+                          int morton_bits, atr_tree** out);
+
+/* The triangle-splitting form: DeviceBuildTree with routing indices
+ * (LBVH.h:108-125, buildTreeTSKernel, LBVH.cu:320-428) and
+ * DeviceCalculateInternalNodeBoundingBoxes (LBVH.h:153-155, LBVH.cu:572-669):
+ * leaf i gets box d_routing[i] of (d_bbmin, d_bbmax) and that box's triangle
+ * d_tri[d_routing[i]] as its data index; num_boxes sorted keys.  Bound to
+ * `scene` as atr_lbvh_build_boxes binds it. */
+atr_status atr_build_tree_boxes(atr_scene* scene, const void* d_sorted_keys,
+                                const uint32_t* d_routing, const float* d_bbmin,
+                                const float* d_bbmax, const int32_t* d_tri, uint32_t num_boxes,
+                                int morton_bits, atr_tree** out);
+
+/* One TRBVH restructuring pass over every node with at least `gamma`
+ * triangles: DeviceTreeletReestructureOptimizer (BVHRT-Core/src/TRBVH.h:42-46,
+ * treeletReestructureKernel, TRBVH.cu:329-480).  TRBVHOptimizer runs it at
+ * gamma = n, 2n, 4n, ... (TRBVHOptimizer.cpp:91,105).  treelet_size 7,
+ * gamma >= treelet_size. */
+atr_status atr_trbvh_pass(atr_tree* tree, int treelet_size, int gamma, float ci, float ct);
+
+/* One ATRBVH restructuring pass: DeviceAgglomerativeTreeletOptimizer /
+ * DeviceAgglomerativeSmallTreeletOptimizer (BVHRT-Core/src/AgglomerativeTreelet.h:39-75,
+ * AgglomerativeTreelet.cu:214-595).  3 <= treelet_size <= 32,
+ * gamma >= treelet_size. */
+atr_status atr_atrbvh_pass(atr_tree* tree, int treelet_size, int gamma, float ci, float ct);
+
+/* DeviceCopyTree + DeviceCalculateCosts (BVHRT-Core/src/TreeCollapser.h:19-42,
+ * copyTreeKernel + calculateCostKernel, TreeCollapser.cu:13-93): the tree
+ * copied into a new collapsed tree and every node's SAH cost and collapse
+ * decision computed.  The result is unfinished until atr_collapse_lists runs
+ * on it (every other call on it fails until then); `tree` must not change in
+ * between. */
+atr_status atr_collapse_costs(atr_tree* tree, float ci, float ct, atr_collapsed** out);
+
+/* DeviceCollapseTree (TreeCollapser.h:60-62, collapseTreeKernel,
+ * TreeCollapser.cu:173-237): the leaf lists of the nodes the cost stage chose,
+ * which finishes `c`.  `tree` is the tree atr_collapse_costs read.  *sah
+ * (nullable) as atr_collapse gives it. */
+atr_status atr_collapse_lists(atr_tree* tree, atr_collapsed* c, float* sah);
+
 /* SoABVHTree::SAH (BVHRT-Core/src/SoABVHTree.cpp:50-74), same summation order;
  * blocking (downloads the area array). */
 atr_status atr_tree_sah(atr_tree* tree, float ci, float ct, float* out);

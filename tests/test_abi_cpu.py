@@ -61,6 +61,14 @@ def test_argument_validation_without_device():
     assert L.atr_sync(None) == 1
     assert L.atr_scene_create(None, None, 10, None, None, ctypes.byref(out)) == 1
     assert L.atr_sort_pairs(None, 32, None, None, 0) == 1
+    # the stage calls
+    assert L.atr_build_tree(None, None, None, 10, 30, ctypes.byref(out)) == 1
+    assert L.atr_build_tree_boxes(None, None, None, None, None, None, 10, 30,
+                                  ctypes.byref(out)) == 1
+    assert L.atr_trbvh_pass(None, 7, 7, 1.2, 1.0) == 1
+    assert L.atr_atrbvh_pass(None, 9, 9, 1.2, 1.0) == 1
+    assert L.atr_collapse_costs(None, 1.2, 1.0, ctypes.byref(out)) == 1
+    assert L.atr_collapse_lists(None, None, None) == 1
     assert b"NULL" in L.atr_last_error() or len(L.atr_last_error()) > 0
     # destroy of NULL handles is a no-op
     assert L.atr_tree_destroy(None) == 0 and L.atr_scene_destroy(None) == 0
