@@ -162,8 +162,15 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     }
     while (cur >= 0) {
         // block-local nodes meet in LDS (workgroup scope), others globally
-        const uint32_t old =
-            local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
+        uint32_t old;
+        if (local) {
+            old = arrive_local(&s_cnt[cur & 255]);
+        } else {
+            // a sibling already there (counter 1) is followed at once: its
+            // state was stored before it arrived (see peek_crossing)
+            old = ld<true>(&counters[cur]);
+            if (old == 0u) old = arrive_crossing(&counters[cur]);
+        }
         if (old == 0u) return;
         const int l = nl;
         const int r = nr;

@@ -227,6 +227,21 @@ __device__ __forceinline__ uint64_t exchange_crossing(uint64_t* slot, uint64_t v
     return old;
 }
 
+// Peek before arriving at a crossing slot.  A thread whose sibling has
+// already arrived sees the sibling's value with one device-coherent load and
+// continues without writing the slot or first waiting for its own stores: no
+// third thread arrives there, and the sibling published only after its own
+// stores had completed, so loads issued once the peeked value has returned
+// (everything that depends on it) see them.  This takes the store wait and
+// the atomic off the path of the continuing thread, which is the later of
+// the two and hence the one the chain to the root waits for.  Returns the
+// empty value (all ones) when the sibling has not arrived; the thread then
+// arrives with exchange_crossing as before.
+__device__ __forceinline__ uint64_t peek_crossing(const uint64_t* slot)
+{
+    return __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Block-local test: every leaf under the node (span) and the node index itself
 // lie in this workgroup's block of 2^SHIFT leaves (256 unless stated).
 template <int SHIFT = 8>

@@ -603,7 +603,9 @@ __device__ __forceinline__ int delta_at(const uint64_t* k, int i)
 // sees this block's threads, so its exchange (and the box handed over) lives
 // in LDS with workgroup-scope ordering.  Only splits whose node crosses a
 // block boundary use the global slot, with the cross-XCD hand-over of
-// device.hpp (coherent stores, relaxed agent-scope exchange, coherent loads).
+// device.hpp (coherent stores, relaxed agent-scope exchange, coherent loads;
+// a thread peeks at the slot first and, when its sibling is already there,
+// takes the parent at once).
 // A split s is block-local iff the nearest smaller delta on both sides lies
 // inside the block, i.e. the prefix minimum of delta over [base-1, s-1] and
 // the suffix minimum over [s+1, base+255] are both below delta(s).
@@ -782,10 +784,18 @@ __global__ __launch_bounds__(256) void k_build(KV kv_in, const uint32_t* __restr
             smn = make_float4(sb[0], sb[1], sb[2], 0.0f);
             smx = make_float4(sb[3], sb[4], sb[5], 0.0f);
         } else {
-            st_coh(&bbmin[node], mn);
-            st_coh(&bbmax[node], mx);
-            old = exchange_crossing(&slots[split], mine);
-            if (old == ~0ull) break;  // first arrival: the sibling finishes the parent
+            old = peek_crossing(&slots[split]);
+            if (old == ~0ull) {
+                st_coh(&bbmin[node], mn);
+                st_coh(&bbmax[node], mx);
+                old = exchange_crossing(&slots[split], mine);
+                if (old == ~0ull) break;  // first arrival: the sibling finishes the parent
+            } else {
+                // the sibling is there: this thread makes the parent, so no
+                // other thread reads this box during the launch
+                bbmin[node] = mn;
+                bbmax[node] = mx;
+            }
             const int sib = (int)(uint32_t)old;
             smn = ld<true>(&bbmin[sib]);
             smx = ld<true>(&bbmax[sib]);
@@ -889,8 +899,15 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
         st_coh(&bbmax[node], mx);
     }
     while (cur >= 0) {
-        const uint32_t old =
-            local ? arrive_local(&s_cnt[cur & 255]) : arrive_crossing(&counters[cur]);
+        uint32_t old;
+        if (local) {
+            old = arrive_local(&s_cnt[cur & 255]);
+        } else {
+            // a sibling already there (counter 1) is followed at once: its
+            // box was stored before it arrived (see peek_crossing)
+            old = ld<true>(&counters[cur]);
+            if (old == 0u) old = arrive_crossing(&counters[cur]);
+        }
         if (old == 0u) return;
         const int other = nl == node ? nr : nl;
         const float4 smn = local ? ld<false>(&bbmin[other]) : ld<true>(&bbmin[other]);

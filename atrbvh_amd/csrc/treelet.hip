@@ -814,7 +814,10 @@ __global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int f
                 old = __hip_atomic_exchange(&s_slot[cur & (BLOCK - 1)], mine, __ATOMIC_ACQ_REL,
                                             __HIP_MEMORY_SCOPE_WORKGROUP);
             } else {
-                old = exchange_crossing(&slots[cur], mine);
+                // a sibling already there is taken with one load (no store
+                // wait, no exchange); otherwise arrive as usual
+                old = peek_crossing(&slots[cur]);
+                if (old == kSlotEmpty) old = exchange_crossing(&slots[cur], mine);
             }
             if (old == kSlotEmpty) {
                 cur = -1;
