@@ -505,6 +505,18 @@ class Tree:
         check(lib().atr_tree_download(self.h, ctypes.byref(v)))
         return out
 
+    def upload(self, host: dict):
+        """HostToDeviceTree (BVHTreeInstanceManager.cpp:56-66) into this tree."""
+        arrs = {k: np.ascontiguousarray(host[k], np.int32) for k in ("parent", "left", "right",
+                                                                       "data")}
+        bmin = np.ascontiguousarray(host["bbmin"], np.float32).reshape(-1, 4)
+        bmax = np.ascontiguousarray(host["bbmax"], np.float32).reshape(-1, 4)
+        area = np.ascontiguousarray(host["area"], np.float32)
+        v = TreeView(self.n, 0, _ptr(arrs["parent"], _i32p), _ptr(arrs["left"], _i32p),
+                     _ptr(arrs["right"], _i32p), _ptr(arrs["data"], _i32p), _ptr(bmin),
+                     _ptr(bmax), _ptr(area))
+        check(lib().atr_tree_upload(self.h, ctypes.byref(v)))
+
     def SAH(self, ci: float = 1.2, ct: float = 1.0) -> float:
         out = ctypes.c_float()
         check(lib().atr_tree_sah(self.h, ci, ct, ctypes.byref(out)))

@@ -426,10 +426,60 @@ static atr_status tree_alloc(atr_context* ctx, uint32_t n, atr_tree** out)
     return ATR_OK;
 }
 
+// Before t's parent / left / right / bbmin / bbmax change: the collapsed tree
+// reading them in place gets its own copy (queued on the stream, so it copies
+// the values the collapse saw).
+static atr_status tree_unshare(atr_tree* t)
+{
+    atr_collapsed* c = t->sharer;
+    if (!c) return ATR_OK;
+    atr_context* ctx = t->ctx;
+    const size_t m = 2ull * t->n - 1;
+    int32_t *p = nullptr, *l = nullptr, *r = nullptr;
+    float4 *mn = nullptr, *mx = nullptr;
+    atr_status s = dalloc_t(ctx, m, &p);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &l);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &r);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &mn);
+    if (s == ATR_OK) s = dalloc_t(ctx, m, &mx);
+    hipError_t e = hipSuccess;
+    if (s == ATR_OK) e = hipMemcpyAsync(p, t->parent, m * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (s == ATR_OK && e == hipSuccess)
+        e = hipMemcpyAsync(l, t->left, m * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (s == ATR_OK && e == hipSuccess)
+        e = hipMemcpyAsync(r, t->right, m * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (s == ATR_OK && e == hipSuccess)
+        e = hipMemcpyAsync(mn, t->bbmin, m * 16, hipMemcpyDeviceToDevice, ctx->stream);
+    if (s == ATR_OK && e == hipSuccess)
+        e = hipMemcpyAsync(mx, t->bbmax, m * 16, hipMemcpyDeviceToDevice, ctx->stream);
+    if (s == ATR_OK && e != hipSuccess) s = fail(ATR_ERR_DEVICE, "collapsed tree copy failed");
+    if (s != ATR_OK) {
+        dfree(ctx, p);
+        dfree(ctx, l);
+        dfree(ctx, r);
+        dfree(ctx, mn);
+        dfree(ctx, mx);
+        return s;
+    }
+    c->parent = p;
+    c->left = l;
+    c->right = r;
+    c->bbmin = mn;
+    c->bbmax = mx;
+    c->links_of = nullptr;
+    t->sharer = nullptr;
+    return ATR_OK;
+}
+
 atr_status atr_tree_destroy(atr_tree* t)
 {
     if (!t) return ATR_OK;
     atr_context* ctx = t->ctx;
+    if (atr_collapsed* c = t->sharer) {  // the collapsed tree keeps the arrays
+        c->links_of = nullptr;
+        t->parent = t->left = t->right = nullptr;
+        t->bbmin = t->bbmax = nullptr;
+    }
     dfree(ctx, t->parent);
     dfree(ctx, t->left);
     dfree(ctx, t->right);
@@ -613,6 +663,7 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int gamma
                                    float ci, float ct)
 {
     atr_context* ctx = t->ctx;
+    if (iterations > 0) ATR_TRY(tree_unshare(t));
     const uint32_t n = t->n;
     float* nsah = nullptr;
     uint64_t* slots = nullptr;
@@ -705,6 +756,7 @@ atr_status atr_collapsed_num_triangles(const atr_collapsed* c, uint32_t* n)
 atr_status atr_tree_device_view(atr_tree* t, atr_tree_view* v)
 {
     if (!t || !v) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    ATR_TRY(tree_unshare(t));  // the caller may write through the view
     v->num_triangles = t->n;
     v->root = 0;
     v->parent = t->parent;
@@ -802,6 +854,7 @@ atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
     if (h->num_triangles != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "size mismatch");
     const size_t m = 2ull * t->n - 1;
     ATR_TRY(atr_tree_view_check(h));
+    ATR_TRY(tree_unshare(t));
     ATR_HIP(hipStreamSynchronize(t->ctx->stream));
     ATR_HIP(hipMemcpy(t->parent, h->parent, m * 4, hipMemcpyHostToDevice));
     ATR_HIP(hipMemcpy(t->left, h->left, m * 4, hipMemcpyHostToDevice));
@@ -851,20 +904,34 @@ atr_status atr_tree_sah(atr_tree* t, float ci, float ct, float* out)
 }
 
 // ------------------------------------------------------------------ collapse
-static atr_status collapsed_alloc(atr_context* ctx, uint32_t n, atr_collapsed** out)
+// The collapsed tree of `t`.  Its parent / left / right / bbmin / bbmax are
+// those of t, so when t has no other reader of them the collapsed tree reads
+// t's arrays in place (t->sharer) instead of a copy.
+static atr_status collapsed_alloc(atr_context* ctx, atr_tree* t, atr_collapsed** out)
 {
+    const uint32_t n = t->n;
     atr_collapsed* c = new atr_collapsed();
     c->ctx = ctx;
     c->n = n;
     const size_t m = 2ull * n - 1;
     atr_status s = ATR_OK;
-    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->parent);
-    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->left);
-    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->right);
+    if (!t->sharer) {
+        c->parent = t->parent;
+        c->left = t->left;
+        c->right = t->right;
+        c->bbmin = t->bbmin;
+        c->bbmax = t->bbmax;
+        c->links_of = t;
+        t->sharer = c;
+    } else {
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &c->parent);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &c->left);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &c->right);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &c->bbmin);
+        if (s == ATR_OK) s = dalloc_t(ctx, m, &c->bbmax);
+    }
     if (s == ATR_OK) s = dalloc_t(ctx, m, &c->data);
     if (s == ATR_OK) s = dalloc_t(ctx, m, &c->count);
-    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->bbmin);
-    if (s == ATR_OK) s = dalloc_t(ctx, m, &c->bbmax);
     if (s == ATR_OK) s = dalloc_t(ctx, n, &c->tri_index);
     if (s == ATR_OK) s = dalloc_t(ctx, 4ull * (n - 1), &c->tnodes);
     if (s == ATR_OK) s = dalloc_t(ctx, 6ull * n, &c->ttris);
@@ -894,6 +961,11 @@ atr_status atr_collapsed_destroy(atr_collapsed* c)
 {
     if (!c) return ATR_OK;
     atr_context* ctx = c->ctx;
+    if (c->links_of) {  // the arrays are the tree's
+        c->links_of->sharer = nullptr;
+        c->parent = c->left = c->right = nullptr;
+        c->bbmin = c->bbmax = nullptr;
+    }
     dfree(ctx, c->parent);
     dfree(ctx, c->left);
     dfree(ctx, c->right);
@@ -942,7 +1014,7 @@ static atr_status collapsed_summary(atr_collapsed* c)
 static atr_status collapse_begin(atr_tree* t, atr_collapsed** out)
 {
     atr_collapsed* c = nullptr;
-    ATR_TRY(collapsed_alloc(t->ctx, t->n, &c));
+    ATR_TRY(collapsed_alloc(t->ctx, t, &c));
     c->has_layout = false;
     c->verts = t->scene ? t->scene->verts : nullptr;
     *out = c;
@@ -1221,6 +1293,7 @@ atr_status atr_refit(atr_tree* t, atr_scene* scene)
     if (!scene) scene = t->scene;
     if (!scene) return fail(ATR_ERR_INVALID_ARGUMENT, "no scene given or bound to the tree");
     if (scene->n != t->n) return fail(ATR_ERR_INVALID_ARGUMENT, "scene/tree size mismatch");
+    ATR_TRY(tree_unshare(t));
     uint32_t* counters = nullptr;
     ATR_TRY(dalloc_t(t->ctx, t->n, &counters));
     atr_status s = run_refit(t->ctx, t, scene->verts, counters);

@@ -92,7 +92,8 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 
 // ------------------------------------------------------------------ cost
 // copyTreeKernel + calculateCostKernel (TreeCollapser.cu:13-93) in one
-// bottom-up pass.  flags bit0 = collapsible, bit1 = "a climb leaves this
+// bottom-up pass.  copy_links = false when the collapsed tree reads the
+// tree's link and box arrays in place (they are not written then).  flags bit0 = collapsible, bit1 = "a climb leaves this
 // node" (leaf, or both children deliver and the node is collapsible).  What a
 // parent needs from a child -- triangle count, cost, flags, height -- is one
 // 16-byte record, stored device-coherently only when the parent's hand-over
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     const float4* __restrict__ bbmax, const float* __restrict__ tarea, int32_t* c_parent,
     int32_t* c_left, int32_t* c_right, int32_t* c_data, int32_t* c_count, float4* c_bbmin,
     float4* c_bbmax, int4* st, uint32_t* emit_count, uint32_t* counters,
-    const int2* __restrict__ span, CollapseSummary* summary)
+    const int2* __restrict__ span, CollapseSummary* summary, bool copy_links)
 {
     __shared__ uint32_t s_cnt[256];
     s_cnt[threadIdx.x] = 0u;
@@ -123,12 +124,14 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= N) return;
     if (i + 1 < N) {  // internal node i
-        c_parent[i] = parent[i];
-        c_left[i] = left[i];
-        c_right[i] = right[i];
         c_data[i] = -1;
-        c_bbmin[i] = bbmin[i];
-        c_bbmax[i] = bbmax[i];
+        if (copy_links) {
+            c_parent[i] = parent[i];
+            c_left[i] = left[i];
+            c_right[i] = right[i];
+            c_bbmin[i] = bbmin[i];
+            c_bbmax[i] = bbmax[i];
+        }
     }
     int node = (int)(N - 1 + i);
     int cur = parent[node];
@@ -150,12 +153,14 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     {
         const float4 mn = bbmin[node];
         const float4 mx = bbmax[node];
-        c_parent[node] = cur;
-        c_left[node] = left[node];
-        c_right[node] = right[node];
+        if (copy_links) {
+            c_parent[node] = cur;
+            c_left[node] = left[node];
+            c_right[node] = right[node];
+            c_bbmin[node] = mn;
+            c_bbmax[node] = mx;
+        }
         c_data[node] = data[node];
-        c_bbmin[node] = mn;
-        c_bbmax[node] = mx;
         c_count[node] = 1;
         put_state(st, node, make_int4(1, __float_as_int(ct * box_area4(mn, mx)), 3, 0),
                   !local && cur >= 0);
@@ -385,7 +390,7 @@ atr_status run_collapse_costs(atr_context* ctx, atr_tree* t, atr_collapsed* c, f
             k_collapse_cost<<<div_up(N, 256), 256, 0, ctx->stream>>>(
                 N, ci, ct, t->parent, t->left, t->right, t->data, t->bbmin, t->bbmax, t->area,
                 c->parent, c->left, c->right, c->data, c->count, c->bbmin, c->bbmax, state,
-                emit_count, counters, t->span, c->d_summary);
+                emit_count, counters, t->span, c->d_summary, c->links_of != t);
         });
     dfree(ctx, state);
     dfree(ctx, counters);

@@ -112,6 +112,11 @@ struct atr_tree {
     // name one triangle, so collapsed leaf lists drop repeats, and a refit
     // from the scene's vertices would not give the leaves' boxes
     bool box_leaves = false;
+    // A collapsed tree that reads this tree's parent / left / right / bbmin /
+    // bbmax arrays in place (the collapse would copy them unchanged).  Before
+    // any of them changes it gets its own copy; when this tree is destroyed
+    // it takes them over.
+    atr_collapsed* sharer = nullptr;
 };
 
 // Summary written by the collapse kernels into device memory.
@@ -155,6 +160,9 @@ struct atr_collapsed {
     // were computed from and each internal node's list length
     const atr_tree* stage_tree = nullptr;
     uint32_t* stage_emit_count = nullptr;
+    // the tree whose link and box arrays parent / left / right / bbmin /
+    // bbmax point at (null when this collapsed tree owns them)
+    atr_tree* links_of = nullptr;
 };
 
 // Many independent meshes built as one forest (batched builds).
