@@ -57,8 +57,9 @@ def build_step(scene, treelet="atrbvh", want_sah=False, morton63=True):
     return tree, collapsed, sah
 
 
-PMC_NAMES = {"build_refit63": "k_build<SingleKeys<unsigned long> >",
-             "onesweep64": "k_onesweep<unsigned long, 16>", "encode63": "k_encode<unsigned long>"}
+PMC_NAMES = {"build_refit63": "k_build<SingleKeys<unsigned long>, TriLeaves>",
+             "onesweep64": "k_onesweep<unsigned long, 16>",
+             "encode63": "k_encode<unsigned long, TriLeaves>"}
 
 
 def traffic_from_profiles(kernel: str):
