@@ -237,9 +237,13 @@ __device__ __forceinline__ void form_treelet_bfs(const HalfTreelet& h, int32_t* 
 }
 
 // ------------------------------------------------------------------ ATRBVH
-// Per-half LDS words: internal 32 + leaf 32 + area 32 + boxes 32*6 + spans
-// 32*2 + matrix.
-constexpr int kAggFixedWords = 32 + 32 + 32 + 192 + 64;
+// Per-half LDS words: internal 32 + leaf 32 + area 32, then the slots' boxes
+// as two float4 arrays {min.xyz, span.x} and {max.xyz, span.y} (32 each, so a
+// slot's box and leaf span are two 16-byte reads, and lane k's own slot sits
+// in a distinct bank group), then the distance matrix.  A half's words are a
+// multiple of 4 so every half's float4 arrays stay 16-byte aligned.
+constexpr int kAggFixedWords = 32 + 32 + 32 + 128 + 128;
+constexpr int agg_half_words(int n) { return (kAggFixedWords + n * (n - 1) / 2 + 3) & ~3; }
 
 // Returns the SAH the root holds afterwards (the agglomerated cost when the
 // treelet is rewritten, else root_sah), the same in every lane of the half.
@@ -255,11 +259,11 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
                                                int2* span, int4* rec)
 {
     HalfTreelet h{(int32_t*)lds, (int32_t*)lds + 32, lds + 64};
-    float* box = lds + 96;
-    int2* sp = (int2*)(lds + 288);
+    float4* smn = (float4*)(lds + 96);   // {min.xyz, span.x as int bits}
+    float4* smx = (float4*)(lds + 224);  // {max.xyz, span.y as int bits}
     float* D = lds + kAggFixedWords;
     if constexpr (BFS)  // the box words hold the heap positions until the boxes load
-        form_treelet_bfs<COH>(h, (int32_t*)box, hl, root, n, first_leaf, rec);
+        form_treelet_bfs<COH>(h, (int32_t*)smn, hl, root, n, first_leaf, rec);
     else
         form_treelet_rec<COH>(h, hl, root, n, first_leaf, rec);
 
@@ -275,22 +279,17 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
         sah = ld<COH>(&nsah[node]);
         const float4 a = ld<COH>(&bbmin[node]);
         const float4 b = ld<COH>(&bbmax[node]);
-        box[hl * 6 + 0] = a.x;
-        box[hl * 6 + 1] = a.y;
-        box[hl * 6 + 2] = a.z;
-        box[hl * 6 + 3] = b.x;
-        box[hl * 6 + 4] = b.y;
-        box[hl * 6 + 5] = b.z;
-        sp[hl] = ld<COH>(&span[node]);
+        const int2 q = ld<COH>(&span[node]);
+        smn[hl] = make_float4(a.x, a.y, a.z, __int_as_float(q.x));
+        smx[hl] = make_float4(b.x, b.y, b.z, __int_as_float(q.y));
     }
     // calculateDistancesMatrix (AgglomerativeTreelet.cu:18-63): every pair i > j
     const int tsize = n * (n - 1) / 2;
     for (int p = hl; p < tsize; p += 32) {
         const int pr = trm_row(p);
-        const float* bi = box + pr * 6;
-        const float* bj = box + (p - (pr * (pr - 1)) / 2) * 6;
-        D[p] = union_area(bi[0], bi[1], bi[2], bi[3], bi[4], bi[5], bj[0], bj[1], bj[2], bj[3],
-                          bj[4], bj[5]);
+        const int pc = p - (pr * (pr - 1)) / 2;
+        const float4 ia = smn[pr], ib = smx[pr], ja = smn[pc], jb = smx[pc];
+        D[p] = union_area(ia.x, ia.y, ia.z, ib.x, ib.y, ib.z, ja.x, ja.y, ja.z, jb.x, jb.y, jb.z);
     }
 
     const bool upper = upper_half();
@@ -339,23 +338,20 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
         const int ri = bcast_half(node, c0, c1, upper);
         const float ali = bcast_half(my_area, r0, r1, upper);
         const float ari = bcast_half(my_area, c0, c1, upper);
-        float amn[3], amx[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            amn[k] = fminf(box[c * 6 + k], box[r * 6 + k]);
-            amx[k] = fmaxf(box[c * 6 + 3 + k], box[r * 6 + 3 + k]);
-        }
-        const int2 spr = sp[r], spc = sp[c];
-        const int2 spm = make_int2(min(spr.x, spc.x), max(spr.y, spc.y));
+        // the merged box (min / max of slots c and r, in that operand order)
+        // and leaf span, in every lane
+        const float4 ca = smn[c], cb = smx[c], ra = smn[r], rb = smx[r];
+        const float amn0 = fminf(ca.x, ra.x), amn1 = fminf(ca.y, ra.y), amn2 = fminf(ca.z, ra.z);
+        const float amx0 = fmaxf(cb.x, rb.x), amx1 = fmaxf(cb.y, rb.y), amx2 = fmaxf(cb.z, rb.z);
+        const float4 mmn = make_float4(
+            amn0, amn1, amn2, __int_as_float(min(__float_as_int(ra.w), __float_as_int(ca.w))));
+        const float4 mmx = make_float4(
+            amx0, amx1, amx2, __int_as_float(max(__float_as_int(rb.w), __float_as_int(cb.w))));
         if (hl == c) {
             node = h.internal[L - 1];
-            sp[c] = spm;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                box[c * 6 + k] = amn[k];
-                box[c * 6 + 3 + k] = amx[k];
-            }
-            const float ar = box_area(amn[0], amn[1], amn[2], amx[0], amx[1], amx[2]);
+            smn[c] = mmn;
+            smx[c] = mmx;
+            const float ar = box_area(amn0, amn1, amn2, amx0, amx1, amx2);
             const float v = s + ci * ar;
             h.area[L] = v;
             sah = v;
@@ -364,48 +360,37 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
         const int lastIndex = bcast_half(node, L, L, upper);
         const float sahLast = bcast_half(sah, L, L, upper);
         const float areaLast = bcast_half(my_area, L, L, upper);
-        float bmn[3], bmx[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            bmn[k] = box[L * 6 + k];
-            bmx[k] = box[L * 6 + 3 + k];
-        }
-        const int2 spL = sp[L];
+        const float4 la = smn[L], lb = smx[L];
         if (hl == r) {
             node = lastIndex;
             sah = sahLast;
             my_area = areaLast;
-            sp[r] = spL;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                box[r * 6 + k] = bmn[k];
-                box[r * 6 + 3 + k] = bmx[k];
-            }
+            smn[r] = la;
+            smx[r] = lb;
         }
         if (hl == L) {
             node = li;
             h.leaf[L] = ri;
             rec_al = ali;
             rec_ar = ari;
-            sp[L] = spm;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                box[L * 6 + k] = amn[k];
-                box[L * 6 + 3 + k] = amx[k];
-            }
+            smn[L] = mmn;
+            smx[L] = mmx;
         }
-        // updateDistancesMatrix (:152-188)
+        // updateDistancesMatrix (:152-188): the merged cluster (now slot c,
+        // its box in registers) against every other remaining slot
         if (L > 1) {
             const bool cpy = hl < L && hl != r;
             float cp = 0.0f;
             if (cpy) cp = D[trm(L, hl)];
             if (cpy) D[trm(max(r, hl), min(r, hl))] = cp;
             if (hl < L && hl != c) {
-                const int dr = max(c, hl), dc = min(c, hl);
-                const float* bi = box + dr * 6;
-                const float* bj = box + dc * 6;
-                D[trm(dr, dc)] = union_area(bi[0], bi[1], bi[2], bi[3], bi[4], bi[5], bj[0],
-                                            bj[1], bj[2], bj[3], bj[4], bj[5]);
+                const float4 oa = smn[hl], ob = smx[hl];
+                // union_area(box[max(c, hl)], box[min(c, hl)])
+                D[trm(max(c, hl), min(c, hl))] =
+                    c > hl ? union_area(amn0, amn1, amn2, amx0, amx1, amx2, oa.x, oa.y, oa.z, ob.x,
+                                        ob.y, ob.z)
+                           : union_area(oa.x, oa.y, oa.z, ob.x, ob.y, ob.z, amn0, amn1, amn2, amx0,
+                                        amx1, amx2);
             }
         }
     }
@@ -421,12 +406,13 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
             st_coh(&parent[node], ni);
             st_coh(&parent[rr], ni);
             st_coh(&nsah[ni], h.area[hl]);
-            const float4 mn = make_float4(box[hl * 6 + 0], box[hl * 6 + 1], box[hl * 6 + 2], 0.0f);
-            const float4 mx = make_float4(box[hl * 6 + 3], box[hl * 6 + 4], box[hl * 6 + 5], 0.0f);
+            const float4 a = smn[hl], b = smx[hl];
+            const float4 mn = make_float4(a.x, a.y, a.z, 0.0f);
+            const float4 mx = make_float4(b.x, b.y, b.z, 0.0f);
             st_coh(&bbmin[ni], mn);
             st_coh(&bbmax[ni], mx);
             st_coh(&area[ni], box_area4(mn, mx));
-            st_coh(&span[ni], sp[hl]);
+            st_coh(&span[ni], make_int2(__float_as_int(a.w), __float_as_int(b.w)));
             st_coh(&rec[ni], make_int4(node, rr, __float_as_int(rec_al), __float_as_int(rec_ar)));
         }
     }
@@ -738,7 +724,7 @@ __global__ __launch_bounds__(BLOCK) void k_treelet_pass(uint32_t n_leaves, int f
                                                       int half_words)
 {
     constexpr int kShift = PassShift<BLOCK>::value;
-    extern __shared__ float smem[];
+    extern __shared__ __align__(16) float smem[];
     __shared__ uint64_t s_slot[BLOCK];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -1009,7 +995,7 @@ atr_status run_atrbvh_pass(atr_context* ctx, atr_tree* t, int treelet, int gamma
                            float ct, float* nsah, uint64_t* slots, int4* rec, int block)
 {
     ATR_TRY(clear_slots(ctx, slots, t->first_leaf));
-    const int half_words = kAggFixedWords + treelet * (treelet - 1) / 2;
+    const int half_words = agg_half_words(treelet);
     return launch_pass_block<true>(ctx, "atrbvh_pass", block, t, treelet, gamma, ci, ct, nsah,
                                    slots, rec, nullptr, 0, half_words);
 }
