@@ -273,6 +273,9 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
     float sah = 0.0f;
     float my_area = 0.0f;
     float rec_al = 0.0f, rec_ar = 0.0f;
+    // lane k > 0 ends up holding internal node k's right child and SAH
+    int out_right = -1;
+    float out_sah = 0.0f;
     if (hl < n) {
         node = h.leaf[hl];
         my_area = h.area[hl];
@@ -309,16 +312,23 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
         // comparisons picks the smallest lane value and, among equal values,
         // the lane with the smallest 5-bit-reversed index (the last step
         // prefers even lanes, the one before lanes 0 mod 4, ...).  Here: the
-        // minimum value of each half (order-free: xor-16 swizzle, then DPP
-        // row shifts into lane 0), a ballot of the lanes holding it, and the
-        // winning lane picked from the ballot on the scalar unit.
-        float mv = fminf(md, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(md), 0x401F)));
-        mv = min_row<8>(mv);
+        // minimum value of each half (order-free: DPP row shifts into each
+        // 16-lane row's first lane, the half's two rows combined on the
+        // scalar unit), a ballot of the lanes holding it, and the winning
+        // lane picked from the ballot on the scalar unit.
+        float mv = min_row<8>(md);
         mv = min_row<4>(mv);
         mv = min_row<2>(mv);
         mv = min_row<1>(mv);
-        const float m0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mv), 0));
-        const float m1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mv), 32));
+        // distances are areas, never negative, so their bit patterns order as
+        // unsigned integers and the two rows of each half combine on the
+        // scalar unit
+        const uint32_t q0 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(mv), 0);
+        const uint32_t q1 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(mv), 16);
+        const uint32_t q2 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(mv), 32);
+        const uint32_t q3 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(mv), 48);
+        const float m0 = __uint_as_float(q0 < q1 ? q0 : q1);
+        const float m1 = __uint_as_float(q2 < q3 ? q2 : q3);
         const uint32_t w0 = (uint32_t)__ballot(md == m0);
         const uint32_t w1 = (uint32_t)(__ballot(md == m1) >> 32);
         // Each half's minimum index, as wave-uniform values (an idle upper
@@ -353,7 +363,6 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
             smx[c] = mmx;
             const float ar = box_area(amn0, amn1, amn2, amx0, amx1, amx2);
             const float v = s + ci * ar;
-            h.area[L] = v;
             sah = v;
             my_area = ar;
         }
@@ -370,7 +379,8 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
         }
         if (hl == L) {
             node = li;
-            h.leaf[L] = ri;
+            out_right = ri;
+            out_sah = s + ci * box_area(amn0, amn1, amn2, amx0, amx1, amx2);
             rec_al = ali;
             rec_ar = ari;
             smn[L] = mmn;
@@ -396,16 +406,16 @@ __device__ __forceinline__ float atrbvh_treelet(float* lds, int hl, int root, fl
     }
 
     // updateTreelet (:190-212)
-    const float cost = h.area[1];
+    const float cost = bcast_half(out_sah, 1, 1, upper);
     if (cost < root_sah) {
         if (hl >= 1 && hl < n) {
             const int ni = h.internal[hl - 1];
-            const int rr = h.leaf[hl];
+            const int rr = out_right;
             st_coh(&left[ni], node);
             st_coh(&right[ni], rr);
             st_coh(&parent[node], ni);
             st_coh(&parent[rr], ni);
-            st_coh(&nsah[ni], h.area[hl]);
+            st_coh(&nsah[ni], out_sah);
             const float4 a = smn[hl], b = smx[hl];
             const float4 mn = make_float4(a.x, a.y, a.z, 0.0f);
             const float4 mx = make_float4(b.x, b.y, b.z, 0.0f);
