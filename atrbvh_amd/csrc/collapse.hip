@@ -92,22 +92,18 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 
 // ------------------------------------------------------------------ cost
 // copyTreeKernel + calculateCostKernel (TreeCollapser.cu:13-93) in one
-// bottom-up pass.  copy_links = false when the collapsed tree reads the
-// tree's link and box arrays in place (they are not written then).  flags bit0 = collapsible, bit1 = "a climb leaves this
+// bottom-up pass.  flags bit0 = collapsible, bit1 = "a climb leaves this
 // node" (leaf, or both children deliver and the node is collapsible).  What a
 // parent needs from a child -- triangle count, cost, flags, height -- is one
-// 16-byte record, stored device-coherently only when the parent's hand-over
-// crosses workgroups (the other child's thread may then run on another XCD).
-// The copy of the tree's links and boxes does not depend on the climb: thread
-// i copies internal node i and leaf i up front, coalesced; the climb writes
-// only the counts.  A node's area is computed from its box, as the reference
-// does (TreeCollapser.cu:50-51), not taken from the tree's area array.
-__device__ __forceinline__ void put_state(int4* st, int node, int4 v, bool coherent)
-{
-    if (coherent) st_coh(&st[node], v);
-    else st[node] = v;
-}
-
+// 16-byte record: kept in LDS, in a slot named by the parent and the child's
+// side, when the parent is block-local, and stored device-coherently in
+// global memory when the parent's hand-over crosses workgroups (the other
+// child's thread may then run on another XCD).  The copy of the tree's links
+// and boxes (copy_links; skipped when the collapsed tree reads the tree's
+// arrays in place) does not depend on the climb: thread i copies internal
+// node i and leaf i up front, coalesced; the climb writes only the counts.  A
+// node's area is computed from its box, as the reference does
+// (TreeCollapser.cu:50-51), not taken from the tree's area array.
 __global__ __launch_bounds__(256) void k_collapse_cost(
     uint32_t N, float ci, float ct, const int32_t* __restrict__ parent,
     const int32_t* __restrict__ left, const int32_t* __restrict__ right,
@@ -118,6 +114,8 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
     const int2* __restrict__ span, CollapseSummary* summary, bool copy_links)
 {
     __shared__ uint32_t s_cnt[256];
+    // a block-local parent p's two child records: [2 (p & 255) + side]
+    __shared__ int4 s_hand[512];
     s_cnt[threadIdx.x] = 0u;
     __syncthreads();
     const int blk = (int)blockIdx.x;
@@ -162,8 +160,13 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         }
         c_data[node] = data[node];
         c_count[node] = 1;
-        put_state(st, node, make_int4(1, __float_as_int(ct * box_area4(mn, mx)), 3, 0),
-                  !local && cur >= 0);
+    }
+    // this thread's node's record, handed to the parent: in LDS (by parent
+    // and side) when the parent is block-local, in global memory otherwise
+    int4 mine = make_int4(1, __float_as_int(ct * box_area4(bbmin[node], bbmax[node])), 3, 0);
+    if (cur >= 0) {
+        if (local) s_hand[2 * (cur & 255) + (nl == node ? 0 : 1)] = mine;
+        else st_coh(&st[node], mine);
     }
     while (cur >= 0) {
         // block-local nodes meet in LDS (workgroup scope), others globally
@@ -177,10 +180,11 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
             if (old == 0u) old = arrive_crossing(&counters[cur]);
         }
         if (old == 0u) return;
-        const int l = nl;
-        const int r = nr;
-        const int4 a = local ? ld<false>(&st[l]) : ld<true>(&st[l]);
-        const int4 b = local ? ld<false>(&st[r]) : ld<true>(&st[r]);
+        const bool is_left = nl == node;
+        const int4 sib = local ? s_hand[2 * (cur & 255) + (is_left ? 1 : 0)]
+                               : ld<true>(&st[is_left ? nr : nl]);
+        const int4 a = is_left ? mine : sib;
+        const int4 b = is_left ? sib : mine;
         const int cl = a.x, cr = b.x;
         const float kl = __int_as_float(a.y), kr = __int_as_float(b.y);
         const uint32_t fl = (uint32_t)a.z, fr = (uint32_t)b.z;
@@ -207,7 +211,11 @@ __global__ __launch_bounds__(256) void k_collapse_cost(
         cur = pp;
         if (cur >= 0) fetch(cur);
         local = cur >= 0 && block_local(nsp, cur, blk);
-        put_state(st, node, make_int4(count, __float_as_int(cc), (int)f, h), !local && cur >= 0);
+        mine = make_int4(count, __float_as_int(cc), (int)f, h);
+        if (cur >= 0) {
+            if (local) s_hand[2 * (cur & 255) + (nl == node ? 0 : 1)] = mine;
+            else st_coh(&st[node], mine);
+        }
     }
 }
 
