@@ -386,12 +386,12 @@ atr_status run_collapse_costs(atr_context* ctx, atr_tree* t, atr_collapsed* c, f
     uint32_t* counters = nullptr;
     atr_status s = dalloc_t(ctx, nodes, &state);
     if (s == ATR_OK) s = dalloc_t(ctx, N, &counters);
+    // every internal node is finished exactly once, so emit_count[0, N-1)
+    // is written by the cost kernel; only the arrival counters start at zero
     if (s == ATR_OK)
-        s = launch(ctx, "collapse_zero", N * 8.0, [&] {
+        s = launch(ctx, "collapse_zero", N * 4.0, [&] {
             k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
                 counters, 0u, N);
-            k_fill_u32c<<<std::min<uint32_t>(div_up(N, 256), 2048u), 256, 0, ctx->stream>>>(
-                emit_count, 0u, N);
         });
     if (s == ATR_OK)
         s = launch(ctx, "collapse_cost", (double)N * 2.0 * 100.0, [&] {

@@ -196,7 +196,7 @@ struct atr_batch {
     // launch shape (atr_batch_config): triangles per encode thread, keys per
     // onesweep thread
     int encode_items = 16;
-    int sort_items = 16;
+    int sort_items = 8;
     int treelet_block = 128;  // threads per workgroup of the treelet passes
     // captured build sequence
     hipGraph_t graph = nullptr;
@@ -257,7 +257,7 @@ size_t sort_status_words(uint32_t n, int passes);
 atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint32_t* vals,
                     uint32_t n, void* keys_alt, uint32_t* vals_alt, bool vals_identity,
                     const uint32_t* block_hist, uint32_t hist_blocks, const SortWork* work,
-                    bool* result_in_alt, int tile_items = 16);
+                    bool* result_in_alt, int tile_items = 8);
 // Forests (batched builds): per-mesh boxes, mesh-tagged 30-bit keys, build.
 int forest_passes(uint32_t meshes);
 atr_status run_bbox_forest(atr_context* ctx, const float4* verts, const uint32_t* d_seg,

@@ -13,8 +13,12 @@ constexpr int kEncBlock = 256;
 constexpr int kEncItems = 16;
 constexpr int kEncTile = kEncBlock * kEncItems;  // triangles per encode block
 
-constexpr int kSortBlock = 256;
-constexpr int kSortWaves = kSortBlock / 64;
+// Radix-sort tiles hold 8192 keys: 1024 threads x 8 keys (the default) or
+// 512 threads x 16.  At 2M keys that is one tile per CU, all resident at
+// once; against 4096-key tiles of 256 threads a 63-bit pass takes 28 us
+// instead of 37 (a shorter look-back chain, half the tiles to dispatch and
+// more waves per tile to hide the loads).
+constexpr int kSortTile = 8192;
 
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
@@ -427,33 +431,36 @@ __global__ void k_zero_two(uint32_t* a, uint64_t na, uint32_t* b, uint64_t nb)
 
 // ---------------------------------------------------------------- onesweep
 // One stable LSD pass over an 8-bit digit.  Tiles are taken in launch order
-// from an atomic ticket; within a tile each wave ranks its 16 x 64 keys with
+// from an atomic ticket; within a tile each wave ranks its ITEMS x 64 keys with
 // ballot-based peer masks (keys keep their order: tile, wave, item, lane), a
-// per-digit scan across waves gives tile-local positions, and a decoupled
-// look-back over earlier tiles' published counts gives global positions.
-// Keys are reordered through LDS so writes of equal digits are contiguous.
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(kSortBlock) void k_onesweep(
+// per-digit scan across waves gives tile-local positions (threads 0..255, one
+// digit each), and a decoupled look-back over earlier tiles' published counts
+// gives global positions.  Keys are reordered through LDS so writes of equal
+// digits are contiguous.
+template <typename K, int ITEMS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, int shift, const uint32_t* __restrict__ digit_off,
     uint32_t* status, uint32_t* tile_ticket, uint32_t* err, uint32_t spin_limit)
 {
+    static_assert(BLOCK % 256 == 0 && BLOCK * ITEMS == kSortTile, "tile shape");
+    constexpr int WAVES = BLOCK / 64;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave_hist[kSortWaves][256];
+    __shared__ uint32_t s_wave_hist[WAVES][256];
     __shared__ uint32_t s_local[256];
     __shared__ uint32_t s_global[256];
     __shared__ uint32_t s_tmp[4];
-    __shared__ K s_keys[(kSortBlock * ITEMS)];
-    __shared__ uint32_t s_vals[(kSortBlock * ITEMS)];
+    __shared__ K s_keys[(BLOCK * ITEMS)];
+    __shared__ uint32_t s_vals[(BLOCK * ITEMS)];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     if (tid == 0) s_tile = atomic_add_relaxed(tile_ticket, 1u);
-    for (int k = tid; k < kSortWaves * 256; k += kSortBlock) (&s_wave_hist[0][0])[k] = 0u;
+    for (int k = tid; k < WAVES * 256; k += BLOCK) (&s_wave_hist[0][0])[k] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t tile_base = tile * (uint32_t)(kSortBlock * ITEMS);
+    const uint32_t tile_base = tile * (uint32_t)(BLOCK * ITEMS);
     const uint32_t wbase = tile_base + (uint32_t)wave * (64u * ITEMS);
 
     K key[ITEMS];
@@ -494,63 +501,73 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
     }
     __syncthreads();
 
-    // thread == digit from here on
-    const uint32_t d = (uint32_t)tid;
+    // threads 0..255: thread == digit from here on
+    const uint32_t d = (uint32_t)(tid & 255);
+    const bool digit_thread = tid < 256;
     uint32_t count = 0;
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) {
-        const uint32_t c = s_wave_hist[w][d];
-        s_wave_hist[w][d] = count;
-        count += c;
-    }
     uint32_t* st = status + (size_t)tile * 256u;
-    store_relaxed(&st[d], (tile == 0 ? kFlagInc : kFlagAgg) | count);
-    const uint32_t local = block256_exclusive_sum(count, s_tmp, nullptr);
-    s_local[d] = local;
-
-    // Look back over the predecessors' status words for this digit, nearest
-    // first, adding aggregates until an inclusive prefix: the words of up to
-    // kLookWindow tiles are loaded together and consumed in order up to the
-    // first one not yet published, which is then waited for.  Every
-    // published word is final for the value it carries, so the sum is the
-    // one-tile-at-a-time walk's.
-    constexpr int kLookWindow = 8;
-    uint32_t excl = 0;
-    if (tile > 0) {
-        int t = (int)tile - 1;
-        uint32_t spins = 0;
-        bool done = false;
-        while (!done) {
-            uint32_t w[kLookWindow];
+    uint32_t inc = 0;
+    if (digit_thread) {
 #pragma unroll
-            for (int k = 0; k < kLookWindow; ++k)
-                w[k] = t - k >= 0 ? load_relaxed(&status[(size_t)(t - k) * 256u + d]) : 0u;
-            int used = 0;
-            bool stall = false;
-#pragma unroll
-            for (int k = 0; k < kLookWindow; ++k) {
-                if (done || stall || t - k < 0) continue;
-                const uint32_t flag = w[k] & ~kValueMask;
-                if (flag == 0u) {
-                    stall = true;
-                    continue;
-                }
-                excl += w[k] & kValueMask;
-                ++used;
-                if (flag == kFlagInc) done = true;
-            }
-            t -= used;
-            if (!done && stall) {
-                if (++spins > spin_limit) {
-                    atomicOr(err, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = s_wave_hist[w][d];
+            s_wave_hist[w][d] = count;
+            count += c;
         }
-        store_relaxed(&st[d], kFlagInc | (excl + count));
+        store_relaxed(&st[d], (tile == 0 ? kFlagInc : kFlagAgg) | count);
+        inc = wave_inclusive_sum(count);
+        if ((tid & 63) == 63) s_tmp[tid >> 6] = inc;
     }
-    s_global[d] = digit_off[d] + excl;
+    __syncthreads();
+    if (digit_thread) {
+        uint32_t add = 0;
+        for (int k = 0; k < (tid >> 6); ++k) add += s_tmp[k];
+        s_local[d] = add + inc - count;
+
+        // Look back over the predecessors' status words for this digit, nearest
+        // first, adding aggregates until an inclusive prefix: the words of up to
+        // kLookWindow tiles are loaded together and consumed in order up to the
+        // first one not yet published, which is then waited for.  Every
+        // published word is final for the value it carries, so the sum is the
+        // one-tile-at-a-time walk's.
+        constexpr int kLookWindow = 8;
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int t = (int)tile - 1;
+            uint32_t spins = 0;
+            bool done = false;
+            while (!done) {
+                uint32_t w[kLookWindow];
+#pragma unroll
+                for (int k = 0; k < kLookWindow; ++k)
+                    w[k] = t - k >= 0 ? load_relaxed(&status[(size_t)(t - k) * 256u + d]) : 0u;
+                int used = 0;
+                bool stall = false;
+#pragma unroll
+                for (int k = 0; k < kLookWindow; ++k) {
+                    if (done || stall || t - k < 0) continue;
+                    const uint32_t flag = w[k] & ~kValueMask;
+                    if (flag == 0u) {
+                        stall = true;
+                        continue;
+                    }
+                    excl += w[k] & kValueMask;
+                    ++used;
+                    if (flag == kFlagInc) done = true;
+                }
+                t -= used;
+                if (!done && stall) {
+                    if (++spins > spin_limit) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            store_relaxed(&st[d], kFlagInc | (excl + count));
+        }
+        s_global[d] = digit_off[d] + excl;
+    }
     __syncthreads();
 
 #pragma unroll
@@ -564,8 +581,8 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(
         }
     }
     __syncthreads();
-    const uint32_t valid_count = min((uint32_t)(kSortBlock * ITEMS), n - tile_base);
-    for (uint32_t k = (uint32_t)tid; k < valid_count; k += kSortBlock) {
+    const uint32_t valid_count = min((uint32_t)(BLOCK * ITEMS), n - tile_base);
+    for (uint32_t k = (uint32_t)tid; k < valid_count; k += BLOCK) {
         const K kk = s_keys[k];
         const uint32_t dd = (uint32_t)(kk >> shift) & 255u;
         const uint32_t o = s_global[dd] + (k - s_local[dd]);
@@ -998,11 +1015,10 @@ atr_status run_encode_boxes(atr_context* ctx, const float4* bmin, const float4* 
     });
 }
 
-// Look-back status words of a sort with the smallest tile (8 items per
-// thread): enough for every tile shape.
+// Look-back status words of a sort (every tile shape holds kSortTile keys).
 size_t sort_status_words(uint32_t n, int passes)
 {
-    return (size_t)passes * (std::max(div_up(n, (uint64_t)kSortBlock * 8), 1u) * 256u + 64u);
+    return (size_t)passes * (std::max(div_up(n, (uint64_t)kSortTile), 1u) * 256u + 64u);
 }
 
 size_t sort_hist_words(uint32_t n, int passes)
@@ -1025,7 +1041,7 @@ atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint
     if (passes < 1 || passes > kbytes) return fail(ATR_ERR_INTERNAL, "bad sort pass count");
     if (tile_items != 8 && tile_items != 16)
         return fail(ATR_ERR_INVALID_ARGUMENT, "sort tile items must be 8 or 16");
-    const uint32_t tiles = std::max(div_up(n, (uint64_t)kSortBlock * tile_items), 1u);
+    const uint32_t tiles = std::max(div_up(n, (uint64_t)kSortTile), 1u);
     uint32_t* offsets = work ? work->offsets : nullptr;
     uint32_t* status = work ? work->status : nullptr;
     uint32_t* own_hist = nullptr;
@@ -1080,22 +1096,22 @@ atr_status run_sort(atr_context* ctx, int key_bits, int passes, void* keys, uint
         if (key_bits == 64) {
             s = launch(ctx, "onesweep64", bytes, [&] {
                 if (tile_items == 16)
-                    k_onesweep<uint64_t, 16><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    k_onesweep<uint64_t, 16, 512><<<tiles, 512, 0, ctx->stream>>>(
                         (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur],
                         n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
                 else
-                    k_onesweep<uint64_t, 8><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    k_onesweep<uint64_t, 8, 1024><<<tiles, 1024, 0, ctx->stream>>>(
                         (const uint64_t*)kbuf[cur], vin, (uint64_t*)kbuf[1 - cur], vbuf[1 - cur],
                         n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
             });
         } else {
             s = launch(ctx, "onesweep32", bytes, [&] {
                 if (tile_items == 16)
-                    k_onesweep<uint32_t, 16><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    k_onesweep<uint32_t, 16, 512><<<tiles, 512, 0, ctx->stream>>>(
                         (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur],
                         n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
                 else
-                    k_onesweep<uint32_t, 8><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    k_onesweep<uint32_t, 8, 1024><<<tiles, 1024, 0, ctx->stream>>>(
                         (const uint32_t*)kbuf[cur], vin, (uint32_t*)kbuf[1 - cur], vbuf[1 - cur],
                         n, 8 * p, offsets + p * 256, st, ticket, ctx->d_status, ctx->spin_limit);
             });
