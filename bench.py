@@ -58,7 +58,7 @@ def build_step(scene, treelet="atrbvh", want_sah=False, morton63=True):
 
 
 PMC_NAMES = {"build_refit63": "k_build<SingleKeys<unsigned long>, TriLeaves>",
-             "onesweep64": "k_onesweep<unsigned long, 16>",
+             "onesweep64": "k_onesweep<unsigned long, 8, 1024>",
              "encode63": "k_encode<unsigned long, TriLeaves>"}
 
 
