@@ -305,10 +305,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_compact(
     // the newest 16 entries per lane in LDS, 112 more in private memory
     __shared__ uint32_t s_ring[kRing * kTraceBlock];
     LdsStack<kRing, 112, kTraceBlock> stack(s_ring + threadIdx.x);
-    const int lane = (int)__lane_id();
     const uint64_t lt = lanemask_lt();
 
-    bool active = false, exhausted = false;
+    bool active = false;
+    RayQueue q(counter, (uint32_t)num_rays);
     int ray = 0;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
     float hitT = 0.0f, idx = 0.0f, idy = 0.0f, idz = 0.0f, oodx = 0.0f, oody = 0.0f, oodz = 0.0f;
@@ -316,14 +316,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_compact(
     while (true) {
         // fetch rays for idle lanes once a quarter of the wave is idle
         const uint64_t idle = __ballot(!active);
-        if (!exhausted && __popcll(idle) >= 16) {
+        if (!q.exhausted && __popcll(idle) >= 16) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(idle));
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-            if (base >= (uint32_t)num_rays) exhausted = true;
-            if (!active) {
-                const uint32_t k = base + (uint32_t)__popcll(idle & lt);
+            uint32_t first = 0;
+            const uint32_t take = q.take((uint32_t)__popcll(idle), leader, first);
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (!active && rank < take) {
+                const uint32_t k = q.ray_of(first + rank);
                 if (k < (uint32_t)num_rays) {
                     ray = (int)k;
                     const float4 o = rays[2ull * k];
@@ -351,7 +350,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_compact(
             }
         }
         if (__ballot(active) == 0ull) {
-            if (exhausted) break;
+            if (q.exhausted) break;
             continue;
         }
         // internal nodes until every active lane stands at a leaf or is done
@@ -435,7 +434,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_compact(
     }
 }
 
-__global__ void k_zero_counter(uint32_t* p) { *p = 0u; }
+__global__ void k_zero_counter(uint32_t* p) { p[threadIdx.x] = 0u; }
 
 }  // namespace
 
@@ -546,9 +545,9 @@ atr_status run_trace_compact(atr_context* ctx, int num_rays, int any_hit, const 
     const uint32_t grid = std::min<uint32_t>(div_up((uint32_t)num_rays, kTraceBlock),
                                              (uint32_t)ctx->cu_count * 8u);
     uint32_t* counter = nullptr;
-    ATR_TRY(dalloc_t(ctx, 1, &counter));
-    atr_status s = launch(ctx, "trace_compact_zero", 4.0, [&] {
-        k_zero_counter<<<1, 1, 0, ctx->stream>>>(counter);
+    ATR_TRY(dalloc_t(ctx, kRayCounterWords, &counter));
+    atr_status s = launch(ctx, "trace_compact_zero", kRayCounterWords * 4.0, [&] {
+        k_zero_counter<<<1, kRayCounterWords, 0, ctx->stream>>>(counter);
     });
     if (s == ATR_OK)
         s = launch(ctx, any_hit ? "trace_compact_any" : "trace_compact", num_rays * 40.0, [&] {

@@ -287,6 +287,77 @@ __device__ __forceinline__ uint32_t block256_exclusive_sum(uint32_t v, uint32_t*
     return add + inc - v;
 }
 
+// --------------------------------------------------------- ray distribution
+// Rays for persistent traversal waves.  kRayCounters counters, each on its own
+// 256-byte line, hand out 64-ray blocks interleaved over the ray range
+// (counter k serves blocks k, k + 8, k + 16, ...), a chunk of positions per
+// atomic into a wave-local queue from which the wave's idle lanes take rays.
+// A single counter taking at most 64 rays per atomic serialised the launch
+// (4,194,304 rays that end at once took 0.85 ms on one MI355X, 0.18 ms this
+// way); spreading the atomics over eight addresses and taking up to two blocks
+// at a time removes that, and the interleave keeps every counter's share of
+// the image alike.  The chunk shrinks with the rays per wave (about four
+// chunks per wave, 16 to 128 positions) so that few rays still spread over
+// every wave.  A wave
+// starts on counter blockIdx.x % 8 and moves on when its counter runs dry, so
+// every ray is handed out exactly once and the wave stops only after it has
+// seen all eight counters dry.  Counters start at zero.
+constexpr int kRayCounters = 8;
+constexpr uint32_t kRayCounterStride = 64;  // words between counters
+constexpr uint32_t kRayCounterWords = kRayCounters * kRayCounterStride;
+
+struct RayQueue {
+    uint32_t* counters;
+    uint32_t R;
+    int ck;
+    int dry = 0;
+    uint32_t next = 0, end = 0;
+    uint32_t chunk;
+    bool exhausted = false;
+
+    __device__ __forceinline__ RayQueue(uint32_t* c, uint32_t r)
+        : counters(c), R(r), ck((int)(blockIdx.x % kRayCounters))
+    {
+        const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+        chunk = min(128u, max(16u, r / (4u * waves)));
+    }
+    // positions of counter k's sequence: 64 per block it serves
+    __device__ __forceinline__ uint32_t span(int k) const
+    {
+        const uint32_t nb = (R + 63u) / 64u;
+        return nb > (uint32_t)k ? ((nb - 1u - (uint32_t)k) / kRayCounters + 1u) * 64u : 0u;
+    }
+    // ray index of position s of the current counter's sequence (may be >= R
+    // in the last, partial block: such positions carry no ray)
+    __device__ __forceinline__ uint32_t ray_of(uint32_t s) const
+    {
+        return ((s >> 6) * kRayCounters + (uint32_t)ck) * 64u + (s & 63u);
+    }
+    // Wave-uniform: up to `want` positions [first, first + return value) of
+    // the current counter; 0 while moving to another counter.
+    __device__ __forceinline__ uint32_t take(uint32_t want, int leader, uint32_t& first)
+    {
+        if (next == end) {
+            const uint32_t sk = span(ck);
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader)
+                base = atomicAdd(&counters[(uint32_t)ck * kRayCounterStride], chunk);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+            if (base >= sk) {
+                if (++dry == kRayCounters) exhausted = true;
+                else ck = (ck + 1) % kRayCounters;
+                return 0u;
+            }
+            next = base;
+            end = min(base + chunk, sk);
+        }
+        const uint32_t t = min(want, end - next);
+        first = next;
+        next += t;
+        return t;
+    }
+};
+
 // --------------------------------------------------------- traversal stack
 // Per-lane stack of a traversal: the newest RING entries live in an LDS ring
 // laid out [entry][thread] (a wave's lanes touch consecutive words), older

@@ -160,8 +160,8 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ rays, 
 }
 
 // Persistent traversal with dynamic ray fetch (the scheme of the reference's
-// kepler_dynamic_fetch kernel): each wave takes rays from a global counter in
-// batches as soon as a quarter of its lanes are idle, so lanes whose ray
+// kepler_dynamic_fetch kernel): each wave takes rays (from a RayQueue, see
+// device.hpp) as soon as a quarter of its lanes are idle, so lanes whose ray
 // missed early or finished early do not idle while the rest of the wave
 // traverses.  Per ray the traversal is exactly k_trace's (same slab test,
 // near child first, far child pushed, LIFO pops), so every ray gets the same
@@ -182,24 +182,22 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
 {
     __shared__ uint32_t s_ring[kFetchRing * 256];
     LdsStack<kFetchRing, 48, 256> stack(s_ring + threadIdx.x);
-    const int lane = (int)__lane_id();
     const uint64_t lt = lanemask_lt();
     bool active = false;
-    bool exhausted = false;
+    RayQueue q(counter, R);
     uint32_t i = 0;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, tmin = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
     float t = 0.0f, ix = 0.0f, iy = 0.0f, iz = 0.0f, bu = 0.0f, bv = 0.0f;
     int id = -1, node = kDone, sp = 0, top = 0;
     while (true) {
         const uint64_t idle = __ballot(!active);
-        if (!exhausted && __popcll(idle) >= 16) {
+        if (!q.exhausted && __popcll(idle) >= 16) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(idle));
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-            if (base >= R) exhausted = true;
-            if (!active) {
-                const uint32_t k = base + (uint32_t)__popcll(idle & lt);
+            uint32_t first = 0;
+            const uint32_t take = q.take((uint32_t)__popcll(idle), leader, first);
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (!active && rank < take) {
+                const uint32_t k = q.ray_of(first + rank);
                 if (k < R) {
                     i = k;
                     const float4 r0 = rays[2ull * i];
@@ -226,7 +224,7 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
             }
         }
         if (__ballot(active) == 0ull) {
-            if (exhausted) break;
+            if (q.exhausted) break;
             continue;
         }
         // internal nodes until every active lane is at a leaf or done
@@ -303,7 +301,7 @@ __global__ __launch_bounds__(256) void k_trace_fetch(const float4* __restrict__ 
     }
 }
 
-__global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+__global__ void k_zero_words(uint32_t* p) { p[threadIdx.x] = 0u; }
 
 // Closest-hit traversal of a batch's binary trees (the forest): ray r belongs
 // to mesh r / rays_per_mesh and starts at that mesh's root (internal slot
@@ -531,9 +529,9 @@ atr_status run_trace(atr_context* ctx, atr_collapsed* c, const float4* rays, uin
         }
         const uint32_t pgrid = std::min<uint32_t>(grid, (uint32_t)ctx->cu_count * 8u);
         uint32_t* counter = nullptr;
-        ATR_TRY(dalloc_t(ctx, 1, &counter));
-        atr_status s = launch(ctx, "trace_zero", 4.0, [&] {
-            k_zero_word<<<1, 1, 0, ctx->stream>>>(counter);
+        ATR_TRY(dalloc_t(ctx, kRayCounterWords, &counter));
+        atr_status s = launch(ctx, "trace_zero", kRayCounterWords * 4.0, [&] {
+            k_zero_words<<<1, kRayCounterWords, 0, ctx->stream>>>(counter);
         });
         if (s == ATR_OK)
             s = launch(ctx, any_hit ? "trace_any" : "trace", bytes, [&] {
