@@ -192,6 +192,18 @@ def _ptr(a: np.ndarray, t=_f32p):
     return a.ctypes.data_as(t)
 
 
+RAY_BYTES = 32      # float4 (origin, tmin) + float4 (direction, tmax)
+RESULT_BYTES = 16   # int4 (triangle, t bits, u bits, v bits)
+
+
+def _need(buf, nbytes: int, what: str):
+    """The C-ABI takes plain pointers and cannot see a buffer's size: a call
+    whose count runs past a DeviceBuffer is refused here, before any kernel
+    reads or writes beyond the allocation."""
+    if buf is not None and int(nbytes) > buf.nbytes:
+        raise AtrError(1, f"{what}: buffer of {buf.nbytes} bytes, the call needs {int(nbytes)}")
+
+
 # ---------------------------------------------------------------- scenes (host)
 def gen_soup(seed: int, n: int, sigma: float = 0.005) -> np.ndarray:
     v = np.zeros((3 * n, 4), np.float32)
@@ -385,6 +397,8 @@ class Context:
     def trace_compact(self, n: int, d_rays: DeviceBuffer, d_results: DeviceBuffer,
                       layout: "CompactLayout", any_hit: bool = False):
         """The reference's `trace` kernel ABI on a Compact2 layout."""
+        _need(d_rays, n * RAY_BYTES, "rays")
+        _need(d_results, n * RESULT_BYTES, "results")
         check(lib().atr_trace_compact(self.h, n, 1 if any_hit else 0, d_rays.ptr, d_results.ptr,
                                       layout.nodes, None, None, None, layout.tris, None, None,
                                       layout.tri_indices))
@@ -393,12 +407,17 @@ class Context:
                      d_rays: DeviceBuffer):
         e = np.asarray(eye, np.float32)
         b = np.asarray(basis, np.float32).reshape(9)
+        _need(d_rays, w * h * RAY_BYTES, "rays")
         check(lib().atr_rays_pinhole(self.h, _ptr(e), _ptr(b), tan_half, w, h, tmax, d_rays.ptr))
 
     def rays_morton_sort(self, d_rays: DeviceBuffer, n: int, d_out: DeviceBuffer,
                          d_slot_to_id: DeviceBuffer, d_id_to_slot: DeviceBuffer | None = None,
                          d_slot_to_id_in: DeviceBuffer | None = None):
         """RayBuffer::mortonSort (gpu-ray-traversal/src/rt/ray/RayBuffer.cpp:103-163)."""
+        for b, size, what in ((d_rays, RAY_BYTES, "rays"), (d_out, RAY_BYTES, "sorted rays"),
+                              (d_slot_to_id, 4, "slot_to_id"), (d_id_to_slot, 4, "id_to_slot"),
+                              (d_slot_to_id_in, 4, "slot_to_id_in")):
+            _need(b, n * size, what)
         check(lib().atr_rays_morton_sort(self.h, d_rays.ptr, n,
                                          d_slot_to_id_in.ptr if d_slot_to_id_in else None,
                                          d_out.ptr, d_slot_to_id.ptr,
@@ -410,11 +429,17 @@ class Context:
 
     def rays_unsort_results(self, d_sorted: DeviceBuffer, d_slot_to_id: DeviceBuffer, n: int,
                             d_out: DeviceBuffer):
+        _need(d_sorted, n * RESULT_BYTES, "sorted results")
+        _need(d_slot_to_id, n * 4, "slot_to_id")
+        _need(d_out, n * RESULT_BYTES, "results")
         check(lib().atr_rays_unsort_results(self.h, d_sorted.ptr, d_slot_to_id.ptr, n,
                                             d_out.ptr))
 
     def rays_diffuse(self, scene: "Scene", d_primary: DeviceBuffer, d_results: DeviceBuffer,
                      n: int, seed: int, tmax: float, d_rays: DeviceBuffer):
+        _need(d_primary, n * RAY_BYTES, "primary rays")
+        _need(d_results, n * RESULT_BYTES, "primary results")
+        _need(d_rays, n * RAY_BYTES, "rays")
         check(lib().atr_rays_diffuse(self.h, scene.h, d_primary.ptr, d_results.ptr, n, seed, tmax,
                                      d_rays.ptr))
 
@@ -572,6 +597,8 @@ class Collapsed:
         check(lib().atr_collapsed_layout(self.h))
 
     def trace(self, d_rays: DeviceBuffer, n: int, d_results: DeviceBuffer, any_hit=False):
+        _need(d_rays, n * RAY_BYTES, "rays")
+        _need(d_results, n * RESULT_BYTES, "results")
         check(lib().atr_trace(self.h, d_rays.ptr, n, 1 if any_hit else 0, d_results.ptr))
 
     def compact(self, node_offset_size_div: int = 16) -> CompactLayout:
@@ -649,6 +676,8 @@ class Batch:
                 "treelet_block": cfg.treelet_block}
 
     def trace(self, d_rays: DeviceBuffer, rays_per_mesh: int, d_hits: DeviceBuffer):
+        _need(d_rays, self.meshes * rays_per_mesh * RAY_BYTES, "rays")
+        _need(d_hits, self.meshes * rays_per_mesh * RESULT_BYTES, "hits")
         check(lib().atr_batch_trace(self.h, d_rays.ptr, rays_per_mesh, d_hits.ptr))
 
     def hit_adjoint(self, d_rays: DeviceBuffer, d_hits: DeviceBuffer, n: int,

@@ -218,3 +218,39 @@ def test_tree_view_check_rejects_cycles_and_bad_leaves():
 def test_no_alloc_sort_argument_validation_without_device():
     L = ab.lib()
     assert L.atr_sort_pairs_noalloc(None, 32, None, None, 0) == 1
+
+
+class _Buf:
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.ptr = None
+
+
+def test_traversal_wrappers_refuse_short_buffers():
+    """The C-ABI sees plain pointers; the Python layer refuses a count that
+    runs past a DeviceBuffer before anything reaches a kernel (with a null
+    handle, a call that got past the check would fail with another message)."""
+    c = ab.Collapsed.__new__(ab.Collapsed)
+    c.h = None
+    ctx = ab.Context.__new__(ab.Context)
+    ctx.h = None
+    n = 70001
+    with pytest.raises(ab.AtrError, match="rays: buffer of"):
+        c.trace(_Buf(21846 * 32), n, _Buf(n * 16))
+    with pytest.raises(ab.AtrError, match="results: buffer of"):
+        c.trace(_Buf(n * 32), n, _Buf(n * 16 - 1))
+    with pytest.raises(ab.AtrError, match="buffer of"):
+        ctx.trace_compact(n, _Buf(n * 32 - 32), _Buf(n * 16), None)
+    with pytest.raises(ab.AtrError, match="buffer of"):
+        ctx.rays_pinhole((0, 0, 0), np.eye(3), 1.0, 8, 8, 1.0, _Buf(63 * 32))
+    with pytest.raises(ab.AtrError, match="buffer of"):
+        ctx.rays_diffuse(None, _Buf(n * 32), _Buf(n * 16), n, 1, 1.0, _Buf(n * 31))
+    with pytest.raises(ab.AtrError, match="buffer of"):
+        ctx.rays_morton_sort(_Buf(n * 32), n, _Buf(n * 32), _Buf(n * 4 - 4))
+    with pytest.raises(ab.AtrError, match="buffer of"):
+        ctx.rays_unsort_results(_Buf(n * 16), _Buf(n * 4), n, _Buf(n * 8))
+    # sizes that fit pass the check and reach the library, which refuses the
+    # null handle
+    with pytest.raises(ab.AtrError) as e:
+        c.trace(_Buf(n * 32), n, _Buf(n * 16))
+    assert "buffer of" not in str(e.value)

@@ -94,6 +94,32 @@ def test_trace_compact_c1(ctx):
     s.close()
 
 
+@pytest.mark.parametrize("count", [1, 63, 65, 1000])
+def test_trace_compact_every_ray_once_at_any_count(ctx, count):
+    """The reference-ABI traversal on batches of any size: each ray traced
+    once with the oracle's result, nothing written past the batch."""
+    v = ab.gen_soup(C1_SEED, C1_N)
+    s, t, c, rc = _pipeline(ctx, v)
+    lay = c.compact(16)
+    nodes, woop, tri = lay.download(ctx)
+    g = 256
+    d_rays = ctx.buffer(g * g * 32)
+    ctx.rays_pinhole(CAMERA_EYE, CAMERA_BASIS, 1.0, g, g, np.inf, d_rays)
+    rays = np.ascontiguousarray(d_rays.download(np.float32, (g * g, 8))[::5][:count])
+    assert rays.shape == (count, 8)  # the ray buffer holds the whole batch
+    d_in = ctx.to_device(rays)
+    d_res = ctx.to_device(np.full((count + 64, 4), 0x5A5A5A5A, np.int32))
+    ctx.trace_compact(count, d_in, d_res, lay)
+    got = d_res.download(np.int32, (count + 64, 4))
+    want = oracle.trace_compact(nodes, woop, tri, rays)
+    assert np.array_equal(got[:count, 0], want[:, 0])
+    assert np.array_equal(got[:count, 1], want[:, 1])
+    assert np.all(got[count:] == 0x5A5A5A5A) and np.all(got[:count, 2:] == 0x5A5A5A5A)
+    c.close()
+    t.close()
+    s.close()
+
+
 def test_compact_full_size_gmm(ctx):
     """The 2,097,152-triangle GMM scene: layout byte-equal, every 16th of the
     2048 x 2048 primary rays equal to the oracle's Woop traversal."""

@@ -397,3 +397,29 @@ def test_cpp_facade_matches_oracle(tmp_path, opt, morton64):
     assert res["used"] == c.used
     assert np.array_equal(res["c_data"], c.data) and np.array_equal(res["c_count"], c.count)
     assert np.array_equal(res["tri_index"], c.tri_index[: c.used])
+
+
+@pytest.mark.parametrize("count", [1, 5, 63, 64, 65, 511, 1000, 70001])
+def test_trace_every_ray_once_at_any_count(ctx, count):
+    """The persistent traversal hands rays out through interleaved counters
+    and wave-local chunks: every ray of a batch of any size is traced exactly
+    once, with the result it gets in a full batch (and from the oracle), and
+    nothing is written past the batch."""
+    v, s, t, c, sah, ref, rc, rsah = _c1_pipeline(ctx)
+    g = 512
+    r = g * g
+    d_rays = ctx.buffer(r * 32)
+    ctx.rays_pinhole(CAMERA_EYE, CAMERA_BASIS, 1.0, g, g, np.inf, d_rays)
+    rays = d_rays.download(np.float32, (r, 8))
+    # take every 3rd ray so that hits and misses mix in small batches
+    sel = np.ascontiguousarray(rays[::3][:count])
+    assert sel.shape == (count, 8)   # the ray buffer holds the whole batch
+    d_sel = ctx.to_device(sel)
+    d_out = ctx.to_device(np.full((count + 64, 4), 0x7F7F7F7F, np.int32))
+    c.trace(d_sel, count, d_out)
+    got = d_out.download(np.int32, (count + 64, 4))
+    compare_hits(got[:count], oracle.trace(rc, v, sel))
+    assert np.all(got[count:] == 0x7F7F7F7F)
+    c.close()
+    t.close()
+    s.close()
