@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -22,6 +24,28 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 atr_status fail(atr_status code, const std::string& msg)
 {
     g_last_error = msg;
+    return code;
+}
+
+// Called only from a catch handler: rethrows the exception being handled to
+// name it.  Storing the message may itself fail (no memory): the status is
+// returned all the same.
+atr_status fail_from_exception() noexcept
+{
+    atr_status code = ATR_ERR_INTERNAL;
+    try {
+        try {
+            throw;
+        } catch (const std::bad_alloc&) {
+            code = ATR_ERR_OUT_OF_MEMORY;
+            g_last_error = "host memory allocation failed";
+        } catch (const std::exception& e) {
+            g_last_error = std::string("host exception: ") + e.what();
+        } catch (...) {
+            g_last_error = "unknown host exception";
+        }
+    } catch (...) {
+    }
     return code;
 }
 
@@ -138,7 +162,7 @@ const char* atr_last_error(void) { return g_last_error.c_str(); }
 const char* atr_version(void) { return "atrbvh-mi355x 0.1 (gfx950)"; }
 
 atr_status atr_context_create(int device, atr_context** out)
-{
+try {
     if (!out) return fail(ATR_ERR_INVALID_ARGUMENT, "out is NULL");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
@@ -176,9 +200,10 @@ atr_status atr_context_create(int device, atr_context** out)
     *out = ctx;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_context_destroy(atr_context* ctx)
-{
+try {
     if (!ctx) return ATR_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -191,9 +216,10 @@ atr_status atr_context_destroy(atr_context* ctx)
     delete ctx;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_sync(atr_context* ctx)
-{
+try {
     if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     uint32_t st = 0;
@@ -210,69 +236,77 @@ atr_status atr_sync(atr_context* ctx)
     if (st & 4u) return fail(ATR_ERR_INTERNAL, "inconsistent traversal layout counts");
     return fail(ATR_ERR_UNSUPPORTED, "a tree was deeper than the traversal stack");
 }
+ATR_CATCH_ALL
 
 void* atr_context_stream(atr_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 atr_status atr_malloc(atr_context* ctx, size_t bytes, void** dptr)
-{
+try {
     if (!ctx || !dptr) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     return dalloc(ctx, bytes, dptr);
 }
+ATR_CATCH_ALL
 
 atr_status atr_free(atr_context* ctx, void* dptr)
-{
+try {
     if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
     dfree(ctx, dptr);
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_memcpy_h2d(atr_context* ctx, void* dst, const void* src, size_t bytes)
-{
+try {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL");
     if (bytes == 0) return ATR_OK;
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     ATR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_memcpy_d2h(atr_context* ctx, void* dst, const void* src, size_t bytes)
-{
+try {
     if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL");
     if (bytes == 0) return ATR_OK;
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     ATR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_memset_d32(atr_context* ctx, void* dst, uint32_t value, size_t count)
-{
+try {
     if (!ctx || (!dst && count)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL");
     if (count == 0) return ATR_OK;
     ATR_HIP(hipMemsetD32Async((hipDeviceptr_t)dst, (int)value, count, ctx->stream));
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_profile_enable(atr_context* ctx, int on)
-{
+try {
     if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
     ctx->profiling = on != 0;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_profile_reset(atr_context* ctx)
-{
+try {
     if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "ctx is NULL");
     resolve_profile(ctx);
     ctx->prof.clear();
     ctx->prof_index.clear();
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 int atr_profile_count(atr_context* ctx) { return ctx ? (int)ctx->prof.size() : 0; }
 
 atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* launches,
                            double* total_ms, double* bytes)
-{
+try {
     if (!ctx || i < 0 || i >= (int)ctx->prof.size())
         return fail(ATR_ERR_INVALID_ARGUMENT, "bad profile index");
     resolve_profile(ctx);
@@ -283,6 +317,7 @@ atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* laun
     if (bytes) *bytes = r.bytes;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // ------------------------------------------------------------------ scene
 static atr_status scene_finish(atr_context* ctx, atr_scene* s, const float* bbmin,
@@ -306,7 +341,7 @@ static atr_status scene_finish(atr_context* ctx, atr_scene* s, const float* bbmi
 
 atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t n,
                             const float* bbmin, const float* bbmax, atr_scene** out)
-{
+try {
     if (!ctx || !vertices || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a scene needs at least 2 triangles");
     if ((bbmin == nullptr) != (bbmax == nullptr))
@@ -339,10 +374,11 @@ atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t n,
     *out = s;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices, uint32_t n,
                                    const float* bbmin, const float* bbmax, atr_scene** out)
-{
+try {
     if (!ctx || !d_vertices || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a scene needs at least 2 triangles");
     if ((bbmin == nullptr) != (bbmax == nullptr))
@@ -369,9 +405,10 @@ atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices, ui
     *out = s;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_scene_bounds(atr_scene* s, float bbmin[3], float bbmax[3])
-{
+try {
     if (!s || !bbmin || !bbmax) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!s->bbox_on_host) {
         ATR_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -384,22 +421,25 @@ atr_status atr_scene_bounds(atr_scene* s, float bbmin[3], float bbmax[3])
     }
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_scene_vertices(atr_scene* s, float** d_vertices)
-{
+try {
     if (!s || !d_vertices) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     *d_vertices = (float*)s->verts;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_scene_destroy(atr_scene* s)
-{
+try {
     if (!s) return ATR_OK;
     dfree(s->ctx, s->verts);
     dfree(s->ctx, s->d_bbox);
     delete s;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // ------------------------------------------------------------------ trees
 static atr_status tree_alloc(atr_context* ctx, uint32_t n, atr_tree** out)
@@ -472,7 +512,7 @@ static atr_status tree_unshare(atr_tree* t)
 }
 
 atr_status atr_tree_destroy(atr_tree* t)
-{
+try {
     if (!t) return ATR_OK;
     atr_context* ctx = t->ctx;
     if (atr_collapsed* c = t->sharer) {  // the collapsed tree keeps the arrays
@@ -491,9 +531,10 @@ atr_status atr_tree_destroy(atr_tree* t)
     delete t;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out)
-{
+try {
     if (!scene || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (morton_bits != 30 && morton_bits != 63)
         return fail(ATR_ERR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
@@ -539,11 +580,12 @@ atr_status atr_lbvh_build(atr_scene* scene, int morton_bits, atr_tree** out)
     *out = t;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_lbvh_build_boxes(atr_scene* scene, const float* d_bbmin, const float* d_bbmax,
                                 const int32_t* d_tri, uint32_t num_boxes, int morton_bits,
                                 atr_tree** out)
-{
+try {
     if (!scene || !d_bbmin || !d_bbmax || !d_tri || !out)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (morton_bits != 30 && morton_bits != 63)
@@ -595,6 +637,7 @@ atr_status atr_lbvh_build_boxes(atr_scene* scene, const float* d_bbmin, const fl
     *out = t;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // The radix tree + boxes stage over caller-sorted keys (either leaf source).
 static atr_status build_stage(atr_scene* scene, const void* keys, const uint32_t* idx,
@@ -628,7 +671,7 @@ static atr_status build_stage(atr_scene* scene, const void* keys, const uint32_t
 atr_status atr_build_tree(atr_scene* scene, const void* d_sorted_keys,
                           const uint32_t* d_sort_indices, uint32_t num_triangles, int morton_bits,
                           atr_tree** out)
-{
+try {
     if (!scene || !d_sorted_keys || !d_sort_indices || !out)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (num_triangles != scene->n)
@@ -636,20 +679,22 @@ atr_status atr_build_tree(atr_scene* scene, const void* d_sorted_keys,
     return build_stage(scene, d_sorted_keys, d_sort_indices, num_triangles, morton_bits, nullptr,
                        nullptr, nullptr, out);
 }
+ATR_CATCH_ALL
 
 atr_status atr_build_tree_boxes(atr_scene* scene, const void* d_sorted_keys,
                                 const uint32_t* d_routing, const float* d_bbmin,
                                 const float* d_bbmax, const int32_t* d_tri, uint32_t num_boxes,
                                 int morton_bits, atr_tree** out)
-{
+try {
     if (!scene || !d_sorted_keys || !d_routing || !d_bbmin || !d_bbmax || !d_tri || !out)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     return build_stage(scene, d_sorted_keys, d_routing, num_boxes, morton_bits,
                        (const float4*)d_bbmin, (const float4*)d_bbmax, d_tri, out);
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_bind_scene(atr_tree* t, atr_scene* scene)
-{
+try {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
     if (t->box_leaves)
         return fail(ATR_ERR_UNSUPPORTED, "a tree built from primitive boxes keeps its scene");
@@ -657,6 +702,7 @@ atr_status atr_tree_bind_scene(atr_tree* t, atr_scene* scene)
     t->scene = scene;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // `iterations` passes at gamma, 2 gamma, 4 gamma, ...
 static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int gamma, int iterations,
@@ -704,57 +750,63 @@ static atr_status treelet_optimize(atr_tree* t, bool agg, int treelet, int gamma
 }
 
 atr_status atr_trbvh_optimize(atr_tree* t, int treelet_size, int iterations, float ci, float ct)
-{
+try {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
     if (treelet_size != 7)
         return fail(ATR_ERR_UNSUPPORTED, "TRBVH supports treelet size 7 only");
     if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     return treelet_optimize(t, false, treelet_size, treelet_size, iterations, ci, ct);
 }
+ATR_CATCH_ALL
 
 atr_status atr_atrbvh_optimize(atr_tree* t, int treelet_size, int iterations, float ci, float ct)
-{
+try {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
     if (treelet_size < 3 || treelet_size > 32)
         return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
     if (iterations < 0) return fail(ATR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
     return treelet_optimize(t, true, treelet_size, treelet_size, iterations, ci, ct);
 }
+ATR_CATCH_ALL
 
 atr_status atr_trbvh_pass(atr_tree* t, int treelet_size, int gamma, float ci, float ct)
-{
+try {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
     if (treelet_size != 7)
         return fail(ATR_ERR_UNSUPPORTED, "TRBVH supports treelet size 7 only");
     if (gamma < treelet_size) return fail(ATR_ERR_INVALID_ARGUMENT, "gamma must be >= treelet_size");
     return treelet_optimize(t, false, treelet_size, gamma, 1, ci, ct);
 }
+ATR_CATCH_ALL
 
 atr_status atr_atrbvh_pass(atr_tree* t, int treelet_size, int gamma, float ci, float ct)
-{
+try {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "tree is NULL");
     if (treelet_size < 3 || treelet_size > 32)
         return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
     if (gamma < treelet_size) return fail(ATR_ERR_INVALID_ARGUMENT, "gamma must be >= treelet_size");
     return treelet_optimize(t, true, treelet_size, gamma, 1, ci, ct);
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_num_triangles(const atr_tree* t, uint32_t* n)
-{
+try {
     if (!t || !n) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     *n = t->n;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapsed_num_triangles(const atr_collapsed* c, uint32_t* n)
-{
+try {
     if (!c || !n) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     *n = c->n;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_device_view(atr_tree* t, atr_tree_view* v)
-{
+try {
     if (!t || !v) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(tree_unshare(t));  // the caller may write through the view
     v->num_triangles = t->n;
@@ -768,9 +820,10 @@ atr_status atr_tree_device_view(atr_tree* t, atr_tree_view* v)
     v->area = t->area;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_download(atr_tree* t, atr_tree_view* h)
-{
+try {
     if (!t || !h) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     const size_t m = 2ull * t->n - 1;
     ATR_HIP(hipStreamSynchronize(t->ctx->stream));
@@ -785,6 +838,7 @@ atr_status atr_tree_download(atr_tree* t, atr_tree_view* h)
     h->root = 0;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // Every kernel that walks a tree trusts its links, so a tree handed in by a
 // caller is checked in full on the host: the BVHRT index convention (root 0,
@@ -793,7 +847,7 @@ atr_status atr_tree_download(atr_tree* t, atr_tree_view* h)
 // one internal node, parent[] equal to those links, and every node reachable
 // from the root (which rules out cycles among internal nodes).
 atr_status atr_tree_view_check(const atr_tree_view* h)
-{
+try {
     if (!h || !h->parent || !h->left || !h->right || !h->data)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     const uint32_t n = h->num_triangles;
@@ -845,9 +899,10 @@ atr_status atr_tree_view_check(const atr_tree_view* h)
                     "tree view has nodes unreachable from the root (a cycle)");
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
-{
+try {
     if (!t || !h || !h->parent || !h->left || !h->right || !h->data || !h->bbmin || !h->bbmax ||
         !h->area)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -868,10 +923,11 @@ atr_status atr_tree_upload(atr_tree* t, const atr_tree_view* h)
     ATR_HIP(hipMemcpy(t->span, sp.data(), m * sizeof(int2), hipMemcpyHostToDevice));
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // Tree created from a host view (for tests and for callers holding a tree).
 atr_status atr_tree_create_from_host(atr_context* ctx, const atr_tree_view* h, atr_tree** out)
-{
+try {
     if (!ctx || !h || !out || h->num_triangles < 2)
         return fail(ATR_ERR_INVALID_ARGUMENT, "bad argument");
     atr_tree* t = nullptr;
@@ -884,9 +940,10 @@ atr_status atr_tree_create_from_host(atr_context* ctx, const atr_tree_view* h, a
     *out = t;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_sah(atr_tree* t, float ci, float ct, float* out)
-{
+try {
     if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     const size_t m = 2ull * t->n - 1;
     std::vector<float> a(m);
@@ -902,6 +959,7 @@ atr_status atr_tree_sah(atr_tree* t, float ci, float ct, float* out)
     *out = (ci * si + ct * sl) / total;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // ------------------------------------------------------------------ collapse
 // The collapsed tree of `t`.  Its parent / left / right / bbmin / bbmax are
@@ -958,7 +1016,7 @@ static atr_status collapsed_alloc(atr_context* ctx, atr_tree* t, atr_collapsed**
 }
 
 atr_status atr_collapsed_destroy(atr_collapsed* c)
-{
+try {
     if (!c) return ATR_OK;
     atr_context* ctx = c->ctx;
     if (c->links_of) {  // the arrays are the tree's
@@ -990,6 +1048,7 @@ atr_status atr_collapsed_destroy(atr_collapsed* c)
     delete c;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 static atr_status collapsed_finished(const atr_collapsed* c)
 {
@@ -1038,7 +1097,7 @@ static atr_status collapse_finish(atr_collapsed* c, float* sah)
 }
 
 atr_status atr_collapse(atr_tree* t, float ci, float ct, atr_collapsed** out, float* sah)
-{
+try {
     if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     atr_collapsed* c = nullptr;
     ATR_TRY(collapse_begin(t, &c));
@@ -1051,9 +1110,10 @@ atr_status atr_collapse(atr_tree* t, float ci, float ct, atr_collapsed** out, fl
     *out = c;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapse_costs(atr_tree* t, float ci, float ct, atr_collapsed** out)
-{
+try {
     if (!t || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     atr_collapsed* c = nullptr;
     ATR_TRY(collapse_begin(t, &c));
@@ -1067,9 +1127,10 @@ atr_status atr_collapse_costs(atr_tree* t, float ci, float ct, atr_collapsed** o
     *out = c;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapse_lists(atr_tree* t, atr_collapsed* c, float* sah)
-{
+try {
     if (!t || !c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!c->stage_emit_count)
         return fail(ATR_ERR_INVALID_ARGUMENT, "collapsed tree does not wait for its lists");
@@ -1082,9 +1143,10 @@ atr_status atr_collapse_lists(atr_tree* t, atr_collapsed* c, float* sah)
     c->stage_tree = nullptr;
     return collapse_finish(c, sah);
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapsed_stats(atr_collapsed* c, float* sah, int* height, uint32_t* used)
-{
+try {
     if (!c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(collapsed_summary(c));
     if (sah) *sah = c->h_summary.cost_root / c->h_summary.area_root;
@@ -1092,9 +1154,10 @@ atr_status atr_collapsed_stats(atr_collapsed* c, float* sah, int* height, uint32
     if (used) *used = c->h_summary.used;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapsed_download(atr_collapsed* c, atr_collapsed_view* h)
-{
+try {
     if (!c || !h) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(collapsed_summary(c));
     const size_t m = 2ull * c->n - 1;
@@ -1115,11 +1178,12 @@ atr_status atr_collapsed_download(atr_collapsed* c, atr_collapsed_view* h)
     h->used = c->h_summary.used;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // ------------------------------------------------------------------ trace
 atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, int any_hit,
                      int32_t* d_results)
-{
+try {
     if (!c || (num_rays && (!d_rays || !d_results)))
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(atr_collapsed_layout(c));
@@ -1127,9 +1191,10 @@ atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, i
     return run_trace(c->ctx, c, (const float4*)d_rays, num_rays, any_hit, d_results,
                      c->h_summary.height);
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapsed_layout(atr_collapsed* c)
-{
+try {
     if (!c) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(collapsed_finished(c));
     if (c->has_layout) return ATR_OK;
@@ -1140,10 +1205,11 @@ atr_status atr_collapsed_layout(atr_collapsed* c)
     c->has_layout = true;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_collapsed_compact(atr_collapsed* c, int node_offset_size_div,
                                  atr_compact_layout* out)
-{
+try {
     if (!c || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(collapsed_finished(c));
     if (node_offset_size_div != 1 && node_offset_size_div != 16)
@@ -1160,12 +1226,13 @@ atr_status atr_collapsed_compact(atr_collapsed* c, int node_offset_size_div,
     out->node_offset_size_div = c->compact_div;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_trace_compact(atr_context* ctx, int num_rays, int any_hit, const float* rays,
                              int32_t* results, const float* nodesA, const float* nodesB,
                              const float* nodesC, const float* nodesD, const float* trisA,
                              const float* trisB, const float* trisC, const int32_t* triIndices)
-{
+try {
     (void)nodesB;
     (void)nodesC;
     (void)nodesD;
@@ -1177,29 +1244,32 @@ atr_status atr_trace_compact(atr_context* ctx, int num_rays, int any_hit, const 
     return run_trace_compact(ctx, num_rays, any_hit, (const float4*)rays, (int2*)results,
                              (const float4*)nodesA, (const float4*)trisA, triIndices);
 }
+ATR_CATCH_ALL
 
 atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float basis[9],
                             float tan_half_fov, uint32_t width, uint32_t height, float tmax,
                             float* d_rays)
-{
+try {
     if (!ctx || !eye || !basis || !d_rays) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     return run_rays_pinhole(ctx, eye, basis, tan_half_fov, width, height, tmax, (float4*)d_rays);
 }
+ATR_CATCH_ALL
 
 atr_status atr_rays_diffuse(atr_context* ctx, atr_scene* scene, const float* d_primary,
                             const int32_t* d_primary_results, uint32_t num_rays, uint32_t seed,
                             float tmax, float* d_rays)
-{
+try {
     if (!ctx || !scene || !d_primary || !d_primary_results || !d_rays)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     return run_rays_diffuse(ctx, scene->verts, (const float4*)d_primary, d_primary_results,
                             num_rays, seed, tmax, (float4*)d_rays);
 }
+ATR_CATCH_ALL
 
 atr_status atr_rays_morton_sort(atr_context* ctx, const float* d_rays, uint32_t num_rays,
                                 const int32_t* d_slot_to_id_in, float* d_rays_out,
                                 int32_t* d_slot_to_id, int32_t* d_id_to_slot)
-{
+try {
     if (!ctx || !d_rays || !d_rays_out || !d_slot_to_id)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (d_rays_out == d_rays) return fail(ATR_ERR_INVALID_ARGUMENT, "output overlaps input");
@@ -1208,20 +1278,22 @@ atr_status atr_rays_morton_sort(atr_context* ctx, const float* d_rays, uint32_t 
     return run_ray_sort(ctx, (const float4*)d_rays, num_rays, d_slot_to_id_in,
                         (float4*)d_rays_out, d_slot_to_id, d_id_to_slot);
 }
+ATR_CATCH_ALL
 
 atr_status atr_rays_morton_keys(atr_context* ctx, const float* d_rays, uint32_t num_rays,
                                 uint64_t* d_keys, float* d_box)
-{
+try {
     if (!ctx || !d_rays || !d_keys || !d_box)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (num_rays == 0) return ATR_OK;
     return run_ray_keys(ctx, (const float4*)d_rays, num_rays, d_keys, d_box);
 }
+ATR_CATCH_ALL
 
 atr_status atr_rays_unsort_results(atr_context* ctx, const int32_t* d_sorted_results,
                                    const int32_t* d_slot_to_id, uint32_t num_rays,
                                    int32_t* d_results)
-{
+try {
     if (!ctx || !d_sorted_results || !d_slot_to_id || !d_results)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (d_results == d_sorted_results)
@@ -1230,12 +1302,13 @@ atr_status atr_rays_unsort_results(atr_context* ctx, const int32_t* d_sorted_res
     return run_unsort_results(ctx, (const int4*)d_sorted_results, d_slot_to_id, num_rays,
                               (int4*)d_results);
 }
+ATR_CATCH_ALL
 
 // ------------------------------------------------------------------ stages
 atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t n,
                              const float bbmin[3], const float bbmax[3], int morton_bits,
                              void* d_keys, uint32_t* d_indices)
-{
+try {
     if (!ctx || !d_vertices || !bbmin || !bbmax || !d_keys)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (morton_bits != 30 && morton_bits != 63)
@@ -1251,10 +1324,11 @@ atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t
     dfree(ctx, d_bbox);
     return s;
 }
+ATR_CATCH_ALL
 
 atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t* d_values,
                           uint32_t n)
-{
+try {
     if (!ctx || (n && !d_keys)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (key_bits != 32 && key_bits != 64)
         return fail(ATR_ERR_INVALID_ARGUMENT, "key_bits must be 32 or 64");
@@ -1283,9 +1357,10 @@ atr_status atr_sort_pairs(atr_context* ctx, int key_bits, void* d_keys, uint32_t
     dfree(ctx, vtmp);
     return s;
 }
+ATR_CATCH_ALL
 
 atr_status atr_refit(atr_tree* t, atr_scene* scene)
-{
+try {
     if (!t) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (t->box_leaves)
         return fail(ATR_ERR_UNSUPPORTED,
@@ -1300,12 +1375,13 @@ atr_status atr_refit(atr_tree* t, atr_scene* scene)
     dfree(t->ctx, counters);
     return s;
 }
+ATR_CATCH_ALL
 
 // ------------------------------------------------------- differentiation
 atr_status atr_hit_adjoint(atr_context* ctx, atr_scene* scene, const float* d_rays,
                            const int32_t* d_hits, uint32_t num_rays, const float* d_grad_tuv,
                            const uint32_t* d_index, uint32_t num_grad_vertices, float* d_grad)
-{
+try {
     if (!ctx || !scene || !d_rays || !d_hits || !d_grad_tuv || !d_grad)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (scene->ctx != ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "scene of another context");
@@ -1317,22 +1393,24 @@ atr_status atr_hit_adjoint(atr_context* ctx, atr_scene* scene, const float* d_ra
                            (const int4*)d_hits, num_rays, d_grad_tuv, d_index, num_grad_vertices,
                            d_grad);
 }
+ATR_CATCH_ALL
 
 atr_status atr_depth_loss(atr_context* ctx, const int32_t* d_hits, const int32_t* d_target_hits,
                           uint32_t num_rays, int same_triangle, float* d_grad_tuv,
                           double* d_loss)
-{
+try {
     if (!ctx || !d_hits || !d_target_hits || !d_grad_tuv || !d_loss)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (num_rays == 0) return ATR_OK;
     return run_depth_loss(ctx, (const int4*)d_hits, (const int4*)d_target_hits, num_rays,
                           same_triangle ? 1 : 0, d_grad_tuv, d_loss);
 }
+ATR_CATCH_ALL
 
 atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_vertices,
                          float* d_grad, float lr, float max_step, const uint32_t* d_index,
                          atr_scene* scene)
-{
+try {
     if (!ctx || !d_mesh_vertices || !d_index || !scene)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (scene->ctx != ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "scene of another context");
@@ -1340,12 +1418,14 @@ atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_
     return run_mesh_step(ctx, (float4*)d_mesh_vertices, num_vertices, d_grad, lr, max_step,
                          d_index, scene->n, scene->verts);
 }
+ATR_CATCH_ALL
 
 atr_status atr_tree_sah_device(atr_tree* t, float ci, float ct, float* d_out)
-{
+try {
     if (!t || !d_out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     return run_sah_parallel(t->ctx, nullptr, 1, t->n, (int)t->first_leaf, t->area, ci, ct, d_out);
 }
+ATR_CATCH_ALL
 
 // --------------------------------------------------------- batched builds
 static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_device,
@@ -1437,23 +1517,26 @@ static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_
 
 atr_status atr_batch_create(atr_context* ctx, const float* vertices, const uint32_t* tri_offsets,
                             uint32_t num_meshes, const float* bboxes, atr_batch** out)
-{
+try {
     return batch_create(ctx, vertices, false, tri_offsets, num_meshes, bboxes, out);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_create_device(atr_context* ctx, const float* d_vertices,
                                    const uint32_t* tri_offsets, uint32_t num_meshes,
                                    const float* bboxes, atr_batch** out)
-{
+try {
     return batch_create(ctx, d_vertices, true, tri_offsets, num_meshes, bboxes, out);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_vertices(atr_batch* b, float** d_vertices)
-{
+try {
     if (!b || !d_vertices) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     *d_vertices = (float*)b->verts;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 // The whole batched build as one stream-ordered sequence of kernels over
 // persistent buffers: no allocation, no memset node, no host synchronisation,
@@ -1494,7 +1577,7 @@ static void batch_drop_graph(atr_batch* b)
 
 atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float ci, float ct,
                            int use_graph)
-{
+try {
     if (!b) return fail(ATR_ERR_INVALID_ARGUMENT, "batch is NULL");
     if (treelet_size < 3 || treelet_size > 32)
         return fail(ATR_ERR_INVALID_ARGUMENT, "ATRBVH treelet size must be in [3, 32]");
@@ -1537,9 +1620,10 @@ atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float
     b->built = true;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_set_config(atr_batch* b, const atr_batch_config* cfg)
-{
+try {
     if (!b || !cfg) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if ((cfg->encode_items != 8 && cfg->encode_items != 16) ||
         (cfg->sort_items != 8 && cfg->sort_items != 16))
@@ -1555,18 +1639,20 @@ atr_status atr_batch_set_config(atr_batch* b, const atr_batch_config* cfg)
     b->treelet_block = tb;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_get_config(const atr_batch* b, atr_batch_config* cfg)
-{
+try {
     if (!b || !cfg) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     cfg->encode_items = b->encode_items;
     cfg->sort_items = b->sort_items;
     cfg->treelet_block = b->treelet_block;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out)
-{
+try {
     if (!b || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     atr_context* ctx = b->ctx;
@@ -1576,18 +1662,20 @@ atr_status atr_batch_sah(atr_batch* b, float ci, float ct, float* out)
     ATR_HIP(hipStreamSynchronize(ctx->stream));
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_sah_device(atr_batch* b, float ci, float ct, float* d_out)
-{
+try {
     if (!b || !d_out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     return run_sah_parallel(b->ctx, b->d_seg, b->meshes, 0, (int)b->forest->first_leaf,
                             b->forest->area, ci, ct, d_out);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_trace(atr_batch* b, const float* d_rays, uint32_t rays_per_mesh,
                            int32_t* d_hits)
-{
+try {
     if (!b || !d_rays || !d_hits) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     const uint64_t r = (uint64_t)rays_per_mesh * b->meshes;
@@ -1596,33 +1684,37 @@ atr_status atr_batch_trace(atr_batch* b, const float* d_rays, uint32_t rays_per_
     return run_trace_forest(b->ctx, b->forest, b->d_seg, b->verts, (const float4*)d_rays,
                             (uint32_t)r, rays_per_mesh, d_hits);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_hit_adjoint(atr_batch* b, const float* d_rays, const int32_t* d_hits,
                                  uint32_t num_rays, const float* d_grad_tuv, float* d_grad)
-{
+try {
     if (!b || !d_rays || !d_hits || !d_grad_tuv || !d_grad)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (num_rays == 0) return ATR_OK;
     return run_hit_adjoint(b->ctx, b->verts, b->n, (const float4*)d_rays, (const int4*)d_hits,
                            num_rays, d_grad_tuv, nullptr, 3u * b->n, d_grad);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr, float max_step)
-{
+try {
     if (!b || !d_grad) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     return run_mesh_step(b->ctx, b->verts, 3u * b->n, d_grad, lr, max_step, nullptr, 0, nullptr);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_refit(atr_batch* b)
-{
+try {
     if (!b) return fail(ATR_ERR_INVALID_ARGUMENT, "batch is NULL");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     return run_refit(b->ctx, b->forest, b->verts, b->counters);
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_rebuild(atr_batch* b, const uint32_t* meshes, uint32_t count,
                              int treelet_size, int iterations, float ci, float ct)
-{
+try {
     if (!b || (!meshes && count)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     if (count == 0) return ATR_OK;
@@ -1674,9 +1766,10 @@ atr_status atr_batch_rebuild(atr_batch* b, const uint32_t* meshes, uint32_t coun
     dfree(ctx, d_map);
     return s;
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out)
-{
+try {
     if (!b || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (mesh >= b->meshes) return fail(ATR_ERR_INVALID_ARGUMENT, "mesh index out of range");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
@@ -1692,9 +1785,10 @@ atr_status atr_batch_extract(atr_batch* b, uint32_t mesh, atr_tree** out)
     *out = t;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_batch_destroy(atr_batch* b)
-{
+try {
     if (!b) return ATR_OK;
     atr_context* ctx = b->ctx;
     (void)hipStreamSynchronize(ctx->stream);
@@ -1720,5 +1814,6 @@ atr_status atr_batch_destroy(atr_batch* b)
     delete b;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 }  // extern "C"

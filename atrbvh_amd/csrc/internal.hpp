@@ -28,6 +28,15 @@ atr_status fail(atr_status code, const std::string& msg);
                                                    hipGetErrorString(_e));               \
     } while (0)
 
+// Every C entry point is a function-try-block closed by ATR_CATCH_ALL, so
+// that no C++ exception (a host allocation that fails, a container's length
+// error) unwinds into a C, Go, Java or Python caller: it becomes
+// ATR_ERR_OUT_OF_MEMORY (std::bad_alloc) or ATR_ERR_INTERNAL, with its
+// message in atr_last_error().
+atr_status fail_from_exception() noexcept;
+#define ATR_CATCH_ALL \
+    catch (...) { return ::atr::fail_from_exception(); }
+
 #define ATR_TRY(expr)                                                                    \
     do {                                                                                 \
         atr_status _s = (expr);                                                          \

@@ -91,7 +91,7 @@ using namespace atr;
 
 extern "C" atr_status atr_sort_pairs_noalloc(atr_context* ctx, int key_bits, void* d_keys,
                                              uint32_t* d_values, uint32_t n)
-{
+try {
     if (!ctx || (n && !d_keys)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (key_bits != 32 && key_bits != 64)
         return fail(ATR_ERR_INVALID_ARGUMENT, "key_bits must be 32 or 64");
@@ -122,3 +122,4 @@ extern "C" atr_status atr_sort_pairs_noalloc(atr_context* ctx, int key_bits, voi
     if (in_alt) return fail(ATR_ERR_INTERNAL, "sort ended in the scratch buffers");
     return ATR_OK;
 }
+ATR_CATCH_ALL

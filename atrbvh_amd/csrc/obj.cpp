@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -46,17 +47,19 @@ namespace atr {
 
 namespace {
 
+// The whole file, read to its end (no size taken from seeking, which a
+// directory or a pipe does not give); false if it cannot be opened or read.
 bool read_file(const std::string& path, std::string& out)
 {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return false;
-    std::fseek(f, 0, SEEK_END);
-    const long size = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    out.resize(size > 0 ? (size_t)size : 0);
-    const size_t got = size > 0 ? std::fread(&out[0], 1, (size_t)size, f) : 0;
+    out.clear();
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) out.append(buf, got);
+    const bool ok = !std::ferror(f);
     std::fclose(f);
-    return got == out.size();
+    return ok;
 }
 
 // Lines with '\r' dropped, '\' + newline joining the next line, trailing
@@ -112,14 +115,17 @@ bool parse_char(const char*& p, char c)
     return true;
 }
 
+// Decimal integer (String.cpp:384-399).  The digits accumulate modulo 2^32,
+// the two's-complement wrap of the framework's int32 accumulation, without
+// signed overflow: a run of digits too long for int32 gives the same bits.
 bool parse_int(const char*& p, int32_t& value)
 {
     const char* t = p;
-    int32_t v = 0;
+    uint32_t v = 0;
     const bool neg = (!parse_char(t, '+') && parse_char(t, '-'));
     if (*t < '0' || *t > '9') return false;
-    while (*t >= '0' && *t <= '9') v = v * 10 + (*t++ - '0');
-    value = neg ? -v : v;
+    while (*t >= '0' && *t <= '9') v = v * 10u + (uint32_t)(*t++ - '0');
+    value = (int32_t)(neg ? 0u - v : v);
     p = t;
     return true;
 }
@@ -299,20 +305,17 @@ using namespace atr;
 extern "C" {
 
 atr_status atr_mesh_load_obj(const char* path, atr_mesh** out)
-{
+try {
     if (!path || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
-    atr_mesh* m = new atr_mesh();
-    const atr_status s = load_obj(path, m);
-    if (s != ATR_OK) {
-        delete m;
-        return s;
-    }
-    *out = m;
+    std::unique_ptr<atr_mesh> m(new atr_mesh());
+    ATR_TRY(load_obj(path, m.get()));
+    *out = m.release();
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_mesh_info(const atr_mesh* m, uint32_t* num_triangles, uint32_t* num_positions)
-{
+try {
     if (!m) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     uint64_t tris = 0;
     for (const auto& g : m->groups) tris += g.size() / 3;
@@ -321,9 +324,10 @@ atr_status atr_mesh_info(const atr_mesh* m, uint32_t* num_triangles, uint32_t* n
     if (num_positions) *num_positions = (uint32_t)(m->positions.size() / 3);
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_mesh_soup(const atr_mesh* m, float* vertices)
-{
+try {
     if (!m || !vertices) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     size_t k = 0;
     for (const auto& g : m->groups) {
@@ -341,15 +345,17 @@ atr_status atr_mesh_soup(const atr_mesh* m, float* vertices)
     }
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_mesh_destroy(atr_mesh* m)
-{
+try {
     delete m;
     return ATR_OK;
 }
+ATR_CATCH_ALL
 
 atr_status atr_scene_create_from_mesh(atr_context* ctx, const atr_mesh* m, atr_scene** out)
-{
+try {
     if (!ctx || !m || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     uint32_t n = 0;
     ATR_TRY(atr_mesh_info(m, &n, nullptr));
@@ -367,5 +373,6 @@ atr_status atr_scene_create_from_mesh(atr_context* ctx, const atr_mesh* m, atr_s
         }
     return atr_scene_create(ctx, v.data(), n, mn, mx, out);
 }
+ATR_CATCH_ALL
 
 }  // extern "C"

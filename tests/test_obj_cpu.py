@@ -127,3 +127,36 @@ def test_missing_file_is_an_error():
     with pytest.raises(ab.AtrError) as e:
         ab.Mesh("/nonexistent/x.obj")
     assert e.value.status == 1
+
+
+def test_directory_path_is_an_error(tmp_path):
+    """A directory opens as a FILE* on Linux but cannot be read: the loader
+    reports it instead of sizing a buffer from a seek that gives no size."""
+    with pytest.raises(ab.AtrError) as e:
+        ab.Mesh(str(tmp_path))
+    assert e.value.status == 1 and "cannot read" in str(e.value)
+
+
+def test_mtllib_naming_a_directory_is_ignored(tmp_path):
+    os.mkdir(os.path.join(str(tmp_path), "lib"))
+    obj = "mtllib lib\nusemtl m\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"
+    info, v = _soup(_write(tmp_path, "d.obj", obj))
+    assert info["triangles"] == 1
+    assert np.array_equal(v[:, :3], np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32))
+
+
+def test_long_indices_wrap_like_int32_accumulation(tmp_path):
+    """Indices accumulate digit by digit in 32-bit two's complement
+    (String.cpp:384-399), so 4294967297 = 2^32 + 1 reads as 1 and
+    99999999999 as 1215752191 (out of range: the origin)."""
+    obj = ("v 0 0 0\nv 1 0 0\nv 0 1 0\n"
+           "f 4294967297 2 3\n"
+           "f 99999999999 2 3\n"
+           "f 1 2 3\n")
+    info, v = _soup(_write(tmp_path, "w.obj", obj))
+    assert info["triangles"] == 3
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    assert np.array_equal(v[0:3, :3], v[6:9, :3])
+    assert np.array_equal(v[0:3, :3], P)
+    assert np.array_equal(v[3, :3], np.zeros(3, np.float32))
+    assert np.array_equal(v[4:6, :3], P[1:])
