@@ -23,9 +23,15 @@
  *                 synthetic treelets; tree invariants after optimisation
  *   tie order     closed-form tie rules of find_minimum, find_leaf_to_expand
  *                 and the DP partition choice, on planted ties
- *   parity unpinned: TRBVH rewrite slot order and the reference quirks that
+ *   composition   ATRBVH (n, 1) on n triangles = greedy clustering placed in
+ *                 formTreelet's expansion order; TRBVH (7, 1) on 7 triangles =
+ *                 exhaustive optimum of the DP objective; treelet counts per
+ *                 gamma = n, 2n, 4n; planted mistakes (ORC_MUTATION) fail them
+ *   parity unpinned: the slot order of rewritten nodes (formTreelet's two
+ *                 children of an expanded node, TRBVH's left-then-right
+ *                 hand-out per stack round) and the reference quirks that
  *                 feed later decisions (right-child cost, root nodesSah not
- *                 updated, root triangle count in every subset's DP term)
+ *                 updated)
  *   collapse      recursive cost / leaf-set partition / SAH=cost/area
  *   trace         brute-force Moller-Trumbore over all triangles
  */
