@@ -1184,6 +1184,8 @@ ATR_CATCH_ALL
 atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, int any_hit,
                      int32_t* d_results)
 try {
+    // the ray queue numbers 64-ray blocks in 32 bits
+    if (num_rays > (1u << 31)) return fail(ATR_ERR_INVALID_ARGUMENT, "at most 2^31 rays per call");
     if (!c || (num_rays && (!d_rays || !d_results)))
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     ATR_TRY(atr_collapsed_layout(c));
@@ -1250,7 +1252,11 @@ atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float ba
                             float tan_half_fov, uint32_t width, uint32_t height, float tmax,
                             float* d_rays)
 try {
+    // the kernel numbers rays in 32 bits
+    const uint64_t count = (uint64_t)width * height;
+    if (count > (1ull << 31)) return fail(ATR_ERR_INVALID_ARGUMENT, "at most 2^31 rays per grid");
     if (!ctx || !eye || !basis || !d_rays) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (count == 0) return ATR_OK;
     return run_rays_pinhole(ctx, eye, basis, tan_half_fov, width, height, tmax, (float4*)d_rays);
 }
 ATR_CATCH_ALL
@@ -1261,6 +1267,7 @@ atr_status atr_rays_diffuse(atr_context* ctx, atr_scene* scene, const float* d_p
 try {
     if (!ctx || !scene || !d_primary || !d_primary_results || !d_rays)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (num_rays == 0) return ATR_OK;
     return run_rays_diffuse(ctx, scene->verts, (const float4*)d_primary, d_primary_results,
                             num_rays, seed, tmax, (float4*)d_rays);
 }

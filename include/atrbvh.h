@@ -278,7 +278,8 @@ atr_status atr_collapsed_destroy(atr_collapsed* c);
  * d_rays: device float4[2r] {ox,oy,oz,tmin, dx,dy,dz,tmax}
  *   (gpu-ray-traversal/src/rt/Util.hpp:62-71);
  * d_results: device int32[4r] {triangle id or -1, t bits, u bits, v bits};
- *   a miss stores t = tmax (RayResult, Util.hpp:77-87). */
+ *   a miss stores t = tmax (RayResult, Util.hpp:77-87).
+ * At most 2^31 rays per call (more: ATR_ERR_INVALID_ARGUMENT). */
 atr_status atr_trace(atr_collapsed* c, const float* d_rays, uint32_t num_rays, int any_hit,
                      int32_t* d_results);
 
@@ -321,7 +322,9 @@ atr_status atr_trace_compact(atr_context* ctx, int num_rays, int any_hit, const 
 
 /* Pinhole primary rays, row-major over a w x h grid: origin `eye`, direction
  * normalize(((2(x+.5)/w-1)*tan_half, (2(y+.5)/h-1)*tan_half, 1)) rotated into
- * the frame (right, up, forward) given by `basis` (9 floats, rows). */
+ * the frame (right, up, forward) given by `basis` (9 floats, rows).
+ * w x h at most 2^31 (more: ATR_ERR_INVALID_ARGUMENT); an empty grid writes
+ * nothing. */
 atr_status atr_rays_pinhole(atr_context* ctx, const float eye[3], const float basis[9],
                             float tan_half_fov, uint32_t width, uint32_t height, float tmax,
                             float* d_rays);

@@ -70,6 +70,17 @@ def test_argument_validation_without_device():
     assert L.atr_collapse_costs(None, 1.2, 1.0, ctypes.byref(out)) == 1
     assert L.atr_collapse_lists(None, None, None) == 1
     assert b"NULL" in L.atr_last_error() or len(L.atr_last_error()) > 0
+    # ray-count limits are checked before anything else
+    assert L.atr_trace(None, None, 2**31 + 1, 0, None) == 1
+    assert b"2^31" in L.atr_last_error()
+    eye = (ctypes.c_float * 3)()
+    basis = (ctypes.c_float * 9)()
+    assert L.atr_rays_pinhole(None, eye, basis, ctypes.c_float(1.0), 65536, 32769,
+                              ctypes.c_float(1.0), None) == 1
+    assert b"2^31" in L.atr_last_error()
+    assert L.atr_rays_pinhole(None, eye, basis, ctypes.c_float(1.0), 65536, 32768,
+                              ctypes.c_float(1.0), None) == 1
+    assert b"NULL" in L.atr_last_error()
     # destroy of NULL handles is a no-op
     assert L.atr_tree_destroy(None) == 0 and L.atr_scene_destroy(None) == 0
 
