@@ -342,6 +342,8 @@ static atr_status scene_finish(atr_context* ctx, atr_scene* s, const float* bbmi
 atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t n,
                             const float* bbmin, const float* bbmax, atr_scene** out)
 try {
+    // trees index their 2n - 1 nodes with int32
+    if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 triangles per scene");
     if (!ctx || !vertices || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a scene needs at least 2 triangles");
     if ((bbmin == nullptr) != (bbmax == nullptr))
@@ -379,6 +381,8 @@ ATR_CATCH_ALL
 atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices, uint32_t n,
                                    const float* bbmin, const float* bbmax, atr_scene** out)
 try {
+    // trees index their 2n - 1 nodes with int32
+    if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 triangles per scene");
     if (!ctx || !d_vertices || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (n < 2) return fail(ATR_ERR_INVALID_ARGUMENT, "a scene needs at least 2 triangles");
     if ((bbmin == nullptr) != (bbmax == nullptr))

@@ -105,7 +105,8 @@ atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* laun
  * The vertex array is COPIED to the device (SceneWrapper takes ownership of a
  * host array instead).  bbmin/bbmax NULL => the scene box is computed on the
  * device with SceneConverter's rule, including its running maximum starting
- * at FLT_MIN (gpu-ray-traversal/src/rt/SceneConverter.cpp:28-64). */
+ * at FLT_MIN (gpu-ray-traversal/src/rt/SceneConverter.cpp:28-64).
+ * 2 <= n <= 2^30 - 1 (more: ATR_ERR_UNSUPPORTED). */
 atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t num_triangles,
                             const float* bbmin, const float* bbmax, atr_scene** out);
 /* Same, from vertices already resident on the device (copied device-to-device).

@@ -70,7 +70,10 @@ def test_argument_validation_without_device():
     assert L.atr_collapse_costs(None, 1.2, 1.0, ctypes.byref(out)) == 1
     assert L.atr_collapse_lists(None, None, None) == 1
     assert b"NULL" in L.atr_last_error() or len(L.atr_last_error()) > 0
-    # ray-count limits are checked before anything else
+    # size limits are checked before anything else
+    assert L.atr_scene_create(None, None, 1 << 30, None, None, ctypes.byref(out)) == 4
+    assert L.atr_scene_create_device(None, None, 1 << 30, None, None, ctypes.byref(out)) == 4
+    assert L.atr_scene_create(None, None, (1 << 30) - 1, None, None, ctypes.byref(out)) == 1
     assert L.atr_trace(None, None, 2**31 + 1, 0, None) == 1
     assert b"2^31" in L.atr_last_error()
     eye = (ctypes.c_float * 3)()
