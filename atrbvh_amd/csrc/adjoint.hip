@@ -17,8 +17,6 @@ namespace atr {
 
 namespace {
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 struct V3 {
     float x, y, z;
 };

@@ -13,8 +13,6 @@ namespace atr {
 
 namespace {
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 // Renumbers one mesh of a forest into a standalone tree.
 __global__ __launch_bounds__(256) void k_forest_extract(
     uint32_t s, uint32_t nm, int first_leaf, const int32_t* __restrict__ P,

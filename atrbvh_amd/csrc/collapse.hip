@@ -88,8 +88,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* __res
     }
 }
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 // ------------------------------------------------------------------ cost
 // copyTreeKernel + calculateCostKernel (TreeCollapser.cu:13-93) in one
 // bottom-up pass.  flags bit0 = collapsible, bit1 = "a climb leaves this

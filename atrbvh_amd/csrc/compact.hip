@@ -28,8 +28,6 @@ namespace atr {
 
 namespace {
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 // ------------------------------------------------------------------ layout
 // Subtree counts of emitted nodes, bottom-up from the tree leaves.  Every
 // hand-over uses the device-coherent protocol of device.hpp (this kernel runs

@@ -16,6 +16,9 @@ namespace atr {
 // flags an error (a correct launch never gets close).
 constexpr uint32_t kSpinLimit = 1u << 26;
 
+// Number of b-sized groups that cover a items (a grid size).
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
 // ------------------------------------------------------------------ errors
 void set_error(const std::string& msg);
 atr_status fail(atr_status code, const std::string& msg);

@@ -946,8 +946,6 @@ __global__ __launch_bounds__(256) void k_refit(uint32_t n, int first_leaf,
     }
 }
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 }  // namespace
 
 uint32_t encode_blocks(uint32_t n) { return div_up(n, kEncTile); }

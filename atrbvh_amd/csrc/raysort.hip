@@ -200,8 +200,6 @@ __global__ __launch_bounds__(256) void k_unsort_results(const int4* __restrict__
     if ((uint32_t)id < r) out[id] = sorted[s];
 }
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 }  // namespace
 
 atr_status run_ray_keys(atr_context* ctx, const float4* rays, uint32_t r, uint64_t* keys,

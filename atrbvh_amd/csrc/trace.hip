@@ -493,8 +493,6 @@ __global__ __launch_bounds__(256) void k_rays_diffuse(const float4* __restrict__
     rays[2ull * i + 1] = make_float4(dx, dy, dz, tri >= 0 ? tmax : -1.0f);
 }
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 }  // namespace
 
 atr_status run_trace_forest(atr_context* ctx, const atr_tree* f, const uint32_t* d_seg,

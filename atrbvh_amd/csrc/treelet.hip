@@ -925,8 +925,6 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t count)
         p[i] = v;
 }
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
 
 // Node records {left, right, area(left), area(right)} of every internal node
 // from the SoA arrays.  Internal slots whose links are not node indices (the
