@@ -453,9 +453,10 @@ atr_status atr_batch_vertices(atr_batch* b, float** d_vertices);
 atr_status atr_batch_build(atr_batch* b, int treelet_size, int iterations, float ci, float ct,
                            int use_graph);
 /* Launch shape of batched builds (results do not depend on it): triangles per
- * thread of the encode kernel, 8 or 16, keys per thread of a radix-sort tile
- * of 8192 keys, 8 (1024 threads) or 16 (512 threads) (default 16/8), and
- * threads per workgroup of the treelet passes, 64, 128 or 256 (0 = 128).
+ * thread of the encode kernel, 8 or 16 (default 16); keys per thread of a
+ * radix-sort tile of 8192 keys, 8 (1024 threads) or 16 (512 threads)
+ * (default 8); threads per workgroup of the treelet passes, 64, 128 or 256
+ * (0 = the default, 128).
  * Changing it drops the captured graph.  Chosen per workload by
  * atrbvh_amd.autotune. */
 typedef struct atr_batch_config {
