@@ -13,14 +13,14 @@ Two searches, both measured on the caller's own batch with graph replays:
 """
 from __future__ import annotations
 
+import itertools
 import time
 
 import numpy as np
 
-import itertools
-
-# (encode_items, sort_items, treelet_block)
-LAUNCH_SHAPES = tuple(itertools.product((16, 8), (16, 8), (128, 64, 256)))
+# (encode_items, sort_items, treelet_block); the first is the library's
+# default shape (atr_batch_config), the one gain_vs_default is relative to
+LAUNCH_SHAPES = tuple(itertools.product((16, 8), (8, 16), (128, 64, 256)))
 TREELET_CANDIDATES = ((9, 2), (7, 2), (9, 1), (11, 1), (11, 2), (13, 1))
 
 
