@@ -1443,16 +1443,17 @@ static atr_status batch_create(atr_context* ctx, const float* vertices, bool on_
                                const uint32_t* tri_offsets, uint32_t meshes,
                                const float* bboxes, atr_batch** out)
 {
-    if (!ctx || !vertices || !tri_offsets || !out)
-        return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!vertices || !tri_offsets || !out) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (meshes == 0) return fail(ATR_ERR_INVALID_ARGUMENT, "a batch needs at least one mesh");
     if (tri_offsets[0] != 0) return fail(ATR_ERR_INVALID_ARGUMENT, "tri_offsets[0] must be 0");
+    // in 64 bits: an offset near 2^32 must not wrap the check round
     for (uint32_t m = 0; m < meshes; ++m)
-        if (tri_offsets[m + 1] < tri_offsets[m] + 2)
+        if ((uint64_t)tri_offsets[m + 1] < (uint64_t)tri_offsets[m] + 2u)
             return fail(ATR_ERR_INVALID_ARGUMENT,
                         "mesh " + std::to_string(m) + " has fewer than 2 triangles");
     const uint32_t n = tri_offsets[meshes];
     if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 triangles per batch");
+    if (!ctx) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!on_device) {
         for (size_t i = 0; i < 3ull * n; ++i)
             for (int a = 0; a < 3; ++a)
@@ -1729,6 +1730,8 @@ try {
     if (!b || (!meshes && count)) return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (!b->built) return fail(ATR_ERR_INVALID_ARGUMENT, "the batch has not been built");
     if (count == 0) return ATR_OK;
+    if (count > b->meshes)
+        return fail(ATR_ERR_INVALID_ARGUMENT, "more mesh indices than the batch has meshes");
     atr_context* ctx = b->ctx;
     std::vector<uint32_t> off(count + 1, 0);
     std::vector<uint8_t> seen(b->meshes, 0);

@@ -142,6 +142,17 @@ def test_batch_argument_validation_without_device():
     v = np.zeros((27, 4), np.float32)
     p32 = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
     assert L.atr_batch_create(None, v.ctypes.data, p32, 2, None, ctypes.byref(out)) == 1
+    assert b"NULL" in L.atr_last_error()
+    # offsets are checked before the context: a mesh of fewer than 2 triangles,
+    # also where an offset near 2^32 would wrap a 32-bit check
+    for bad, mesh in (([0, 5, 6], 1), ([0, 4294967295, 5], 1), ([0, 1, 9], 0)):
+        o = np.array(bad, np.uint32)
+        assert L.atr_batch_create(None, v.ctypes.data, o.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_uint32)), 2, None, ctypes.byref(out)) == 1
+        assert f"mesh {mesh} has fewer".encode() in L.atr_last_error()
+    big = np.array([0, 1 << 30], np.uint32)
+    assert L.atr_batch_create(None, v.ctypes.data, big.ctypes.data_as(
+        ctypes.POINTER(ctypes.c_uint32)), 1, None, ctypes.byref(out)) == 4
     assert L.atr_batch_build(None, 9, 2, 1.2, 1.0, 1) == 1
     assert L.atr_batch_destroy(None) == 0
 
