@@ -1320,6 +1320,7 @@ atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t
                              const float bbmin[3], const float bbmax[3], int morton_bits,
                              void* d_keys, uint32_t* d_indices)
 try {
+    if (n >= (1u << 30)) return fail(ATR_ERR_UNSUPPORTED, "at most 2^30-1 triangles per call");
     if (!ctx || !d_vertices || !bbmin || !bbmax || !d_keys)
         return fail(ATR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (morton_bits != 30 && morton_bits != 63)
@@ -1348,13 +1349,14 @@ try {
     void* kalt = nullptr;
     uint32_t* valt = nullptr;
     uint32_t* vtmp = nullptr;
-    ATR_TRY(dalloc(ctx, (size_t)n * key_bits / 8, &kalt));
-    ATR_TRY(dalloc_t(ctx, n, &valt));
     const bool identity = d_values == nullptr;
-    if (identity) ATR_TRY(dalloc_t(ctx, n, &vtmp));
     bool in_alt = false;
-    atr_status s = run_sort(ctx, key_bits, key_bits / 8, d_keys, identity ? vtmp : d_values, n,
-                            kalt, valt, identity, nullptr, 0, nullptr, &in_alt);
+    atr_status s = dalloc(ctx, (size_t)n * key_bits / 8, &kalt);
+    if (s == ATR_OK) s = dalloc_t(ctx, n, &valt);
+    if (s == ATR_OK && identity) s = dalloc_t(ctx, n, &vtmp);
+    if (s == ATR_OK)
+        s = run_sort(ctx, key_bits, key_bits / 8, d_keys, identity ? vtmp : d_values, n, kalt,
+                     valt, identity, nullptr, 0, nullptr, &in_alt);
     if (s == ATR_OK && in_alt) {
         hipError_t e = hipMemcpyAsync(d_keys, kalt, (size_t)n * key_bits / 8,
                                       hipMemcpyDeviceToDevice, ctx->stream);

@@ -366,7 +366,8 @@ atr_status atr_rays_unsort_results(atr_context* ctx, const int32_t* d_sorted_res
 
 /* ------------------------------------------------------- low-level stages */
 /* DeviceGenerateMortonCodes (BVHRT-Core/src/LBVH.h:20-50): keys (u32 for 30,
- * u64 for 63 bits) and indices (identity; may be NULL) on caller buffers. */
+ * u64 for 63 bits) and indices (identity; may be NULL) on caller buffers.
+ * n <= 2^30 - 1 (more: ATR_ERR_UNSUPPORTED). */
 atr_status atr_morton_encode(atr_context* ctx, const float* d_vertices, uint32_t n,
                              const float bbmin[3], const float bbmax[3], int morton_bits,
                              void* d_keys, uint32_t* d_indices);

@@ -71,6 +71,7 @@ def test_argument_validation_without_device():
     assert L.atr_collapse_lists(None, None, None) == 1
     assert b"NULL" in L.atr_last_error() or len(L.atr_last_error()) > 0
     # size limits are checked before anything else
+    assert L.atr_morton_encode(None, None, 1 << 30, None, None, 30, None, None) == 4
     assert L.atr_scene_create(None, None, 1 << 30, None, None, ctypes.byref(out)) == 4
     assert L.atr_scene_create_device(None, None, 1 << 30, None, None, ctypes.byref(out)) == 4
     assert L.atr_scene_create(None, None, (1 << 30) - 1, None, None, ctypes.byref(out)) == 1
