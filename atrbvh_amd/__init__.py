@@ -367,10 +367,15 @@ class Context:
                       d_keys: DeviceBuffer, d_idx: DeviceBuffer | None):
         mn = np.asarray(bbmin, np.float32)
         mx = np.asarray(bbmax, np.float32)
+        _need(d_vertices, n * 48, "vertices")
+        _need(d_keys, n * (8 if bits == 63 else 4), "keys")
+        _need(d_idx, n * 4, "indices")
         check(lib().atr_morton_encode(self.h, d_vertices.ptr, n, _ptr(mn), _ptr(mx), bits,
                                       d_keys.ptr, d_idx.ptr if d_idx is not None else None))
 
     def sort_pairs(self, key_bits: int, d_keys: DeviceBuffer, d_vals: DeviceBuffer | None, n: int):
+        _need(d_keys, n * (key_bits // 8), "keys")
+        _need(d_vals, n * 4, "values")
         check(lib().atr_sort_pairs(self.h, key_bits, d_keys.ptr,
                                    d_vals.ptr if d_vals is not None else None, n))
 

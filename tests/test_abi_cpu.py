@@ -275,6 +275,10 @@ def test_traversal_wrappers_refuse_short_buffers():
         ctx.rays_morton_sort(_Buf(n * 32), n, _Buf(n * 32), _Buf(n * 4 - 4))
     with pytest.raises(ab.AtrError, match="buffer of"):
         ctx.rays_unsort_results(_Buf(n * 16), _Buf(n * 4), n, _Buf(n * 8))
+    with pytest.raises(ab.AtrError, match="keys: buffer of"):
+        ctx.morton_encode(_Buf(n * 48), n, (0, 0, 0), (1, 1, 1), 63, _Buf(n * 4), None)
+    with pytest.raises(ab.AtrError, match="values: buffer of"):
+        ctx.sort_pairs(32, _Buf(n * 4), _Buf(n * 2), n)
     # sizes that fit pass the check and reach the library, which refuses the
     # null handle
     with pytest.raises(ab.AtrError) as e:
