@@ -55,8 +55,11 @@ bool read_file(const std::string& path, std::string& out)
     if (!f) return false;
     out.clear();
     char buf[1 << 16];
-    size_t got;
-    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) out.append(buf, got);
+    for (;;) {
+        const size_t got = std::fread(buf, 1, sizeof buf, f);
+        out.append(buf, got);
+        if (got < sizeof buf) break;  // end of file or a read error
+    }
     const bool ok = !std::ferror(f);
     std::fclose(f);
     return ok;
