@@ -110,7 +110,10 @@ atr_status atr_profile_get(atr_context* ctx, int i, const char** name, int* laun
 atr_status atr_scene_create(atr_context* ctx, const float* vertices, uint32_t num_triangles,
                             const float* bbmin, const float* bbmax, atr_scene** out);
 /* Same, from vertices already resident on the device (copied device-to-device).
- * Coordinates must be finite: host arrays are checked, device arrays are not. */
+ * Coordinates must be finite: host arrays are checked, device arrays are not
+ * (a pass over them would cost a device-to-host round trip per call).  The
+ * treelet passes are compiled on the premise that boxes, areas and SAH values
+ * are finite; a scene with a non-finite coordinate has no defined tree. */
 atr_status atr_scene_create_device(atr_context* ctx, const float* d_vertices,
                                    uint32_t num_triangles, const float* bbmin,
                                    const float* bbmax, atr_scene** out);
@@ -417,7 +420,9 @@ atr_status atr_depth_loss(atr_context* ctx, const int32_t* d_hits, const int32_t
  * shortened to length max_step when it is longer (max_step <= 0: no limit);
  * d_grad is then cleared (d_grad NULL => no step).  The scene's triangle soup
  * is then gathered from the mesh through d_index (uint32[3n]).  Follow with
- * atr_refit to update the tree. */
+ * atr_refit to update the tree.  Nothing here checks that the moved
+ * coordinates stay finite: a step that sends one to inf or NaN leaves the
+ * scene outside what the builders accept (see atr_scene_create_device). */
 atr_status atr_mesh_step(atr_context* ctx, float* d_mesh_vertices, uint32_t num_vertices,
                          float* d_grad, float lr, float max_step, const uint32_t* d_index,
                          atr_scene* scene);
@@ -438,7 +443,8 @@ atr_status atr_tree_sah_device(atr_tree* tree, float ci, float ct, float* d_out)
  * tri_offsets: host uint32[num_meshes + 1] prefix offsets, tri_offsets[0] = 0,
  * every mesh >= 2 triangles, T < 2^30; bboxes: host float[6 * num_meshes]
  * (min xyz, max xyz per mesh) or NULL => every build computes each mesh's box
- * on the device with SceneConverter's rule. */
+ * on the device with SceneConverter's rule.  Coordinates must be finite: host
+ * arrays are checked, device arrays are not. */
 typedef struct atr_batch atr_batch;
 atr_status atr_batch_create(atr_context* ctx, const float* vertices, const uint32_t* tri_offsets,
                             uint32_t num_meshes, const float* bboxes, atr_batch** out);
@@ -446,7 +452,7 @@ atr_status atr_batch_create_device(atr_context* ctx, const float* d_vertices,
                                    const uint32_t* tri_offsets, uint32_t num_meshes,
                                    const float* bboxes, atr_batch** out);
 /* Device pointer of the batch's float4 vertex array (mesh-major); callers may
- * move vertices in place between builds. */
+ * move vertices in place between builds, keeping every coordinate finite. */
 atr_status atr_batch_vertices(atr_batch* b, float** d_vertices);
 /* LBVH (30-bit) + ATRBVH(treelet, iterations) of every mesh, asynchronous.
  * use_graph != 0: the launch sequence is captured once into a HIP graph (per
@@ -486,7 +492,8 @@ atr_status atr_batch_hit_adjoint(atr_batch* b, const float* d_rays, const int32_
                                  uint32_t num_rays, const float* d_grad_tuv, float* d_grad);
 /* Vertex step on the batch's soup: every corner moves by -lr * d_grad
  * (float[9T]), shortened to length max_step when longer (max_step <= 0: no
- * limit); d_grad is then cleared. */
+ * limit); d_grad is then cleared.  As for atr_mesh_step, the moved
+ * coordinates must stay finite for later builds. */
 atr_status atr_batch_step(atr_batch* b, float* d_grad, float lr, float max_step);
 /* Bottom-up refit of every mesh's tree over the batch's current vertices. */
 atr_status atr_batch_refit(atr_batch* b);
