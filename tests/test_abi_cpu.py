@@ -284,3 +284,40 @@ def test_traversal_wrappers_refuse_short_buffers():
     with pytest.raises(ab.AtrError) as e:
         c.trace(_Buf(n * 32), n, _Buf(n * 16))
     assert "buffer of" not in str(e.value)
+
+
+class _OkLib:
+    """Stands in for the library: every call succeeds (exercises the Python
+    side of the wrappers only)."""
+
+    def __getattr__(self, name):
+        return lambda *a: 0
+
+
+def test_traversal_wrappers_accept_exact_sizes(monkeypatch):
+    monkeypatch.setattr(ab, "lib", lambda: _OkLib())
+    c = ab.Collapsed.__new__(ab.Collapsed)
+    c.h = None
+    ctx = ab.Context.__new__(ab.Context)
+    ctx.h = None
+    n = 1000
+    c.trace(_Buf(n * 32), n, _Buf(n * 16))
+    lay = ab.CompactLayout()
+    ctx.trace_compact(n, _Buf(n * 32), _Buf(n * 16), lay)
+    ctx.rays_pinhole((0, 0, 0), np.eye(3), 1.0, 40, 25, 1.0, _Buf(n * 32))
+    s = ab.Scene.__new__(ab.Scene)
+    s.h = None
+    ctx.rays_diffuse(s, _Buf(n * 32), _Buf(n * 16), n, 1, 1.0, _Buf(n * 32))
+    ctx.rays_morton_sort(_Buf(n * 32), n, _Buf(n * 32), _Buf(n * 4), _Buf(n * 4), _Buf(n * 4))
+    ctx.rays_morton_sort(_Buf(n * 32), n, _Buf(n * 32), _Buf(n * 4))
+    ctx.rays_unsort_results(_Buf(n * 16), _Buf(n * 4), n, _Buf(n * 16))
+    ctx.morton_encode(_Buf(n * 48), n, (0, 0, 0), (1, 1, 1), 63, _Buf(n * 8), _Buf(n * 4))
+    ctx.morton_encode(_Buf(n * 48), n, (0, 0, 0), (1, 1, 1), 30, _Buf(n * 4), None)
+    ctx.sort_pairs(64, _Buf(n * 8), _Buf(n * 4), n)
+    ctx.sort_pairs(32, _Buf(n * 4), None, n)
+    b = ab.Batch.__new__(ab.Batch)
+    b.h = None
+    b.meshes = 10
+    b.trace(_Buf(n * 32), n // 10, _Buf(n * 16))
+    with pytest.raises(ab.AtrError, match="hits: buffer of"):
+        b.trace(_Buf(n * 32), n // 10, _Buf(n * 16 - 16))
