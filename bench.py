@@ -57,6 +57,31 @@ def build_step(scene, treelet="atrbvh", want_sah=False, morton63=True):
     return tree, collapsed, sah
 
 
+def timed_builds(ctx, scene, n: int, morton63: bool = True, reps: int = 10,
+                 warmup: int = 2) -> dict:
+    """LBVH + ATRBVH (9, 2) + collapse of `scene` (n triangles): `warmup`
+    untimed builds, then `reps` builds timed back to back between stream
+    synchronisations; the rate and the collapsed tree's SAH."""
+    for _ in range(warmup):
+        t, c, _ = build_step(scene, morton63=morton63)
+        c.close()
+        t.close()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t, c, _ = build_step(scene, morton63=morton63)
+        c.close()
+        t.close()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    t, c, sah = build_step(scene, want_sah=True, morton63=morton63)
+    c.close()
+    t.close()
+    return {"triangles": n, "morton_bits": 63 if morton63 else 30, "reps": reps,
+            "ms_per_build": round(ms, 4), "Mtris_s": round(n / (ms * 1e-3) / 1e6, 2),
+            "sah": round(float(sah), 4)}
+
+
 PMC_NAMES = {"build_refit63": "k_build<SingleKeys<unsigned long>, TriLeaves>",
              "onesweep64": "k_onesweep<unsigned long, 8, 1024>",
              "encode63": "k_encode<unsigned long, TriLeaves>"}
@@ -534,6 +559,18 @@ def main():
     mtris = N_TRIS / (ms_per_step * 1e-3) / 1e6
     log(f"[bench] build {ms_per_step:.3f} ms/step, {mtris:.1f} Mtris/s")
 
+    # the same build of the same scene family at 1,048,576 triangles (the size
+    # the metric names); a failure here is reported in the line, not fatal
+    try:
+        v1m = ab.gen_gmm(SEED, 1 << 20)
+        s1m = ctx.scene(v1m)
+        build_1m = timed_builds(ctx, s1m, 1 << 20, True, reps=max(args.steps, 1))
+        s1m.close()
+        del v1m
+        log(f"[bench] 1M build {build_1m['ms_per_build']} ms, {build_1m['Mtris_s']} Mtris/s")
+    except Exception as e:  # noqa: BLE001 -- keep the headline line
+        build_1m = {"error": f"{type(e).__name__}: {e}"}
+
     # ---------------------------------------------------------------- profiled steps
     ctx.profile(True)
     ctx.profile_reset()
@@ -761,6 +798,7 @@ def main():
         "traversal_layout": {"ms": round(tlayout_ms, 3),
                              "build_plus_layout_Mtris_s": round(
                                  N_TRIS / ((ms_per_step + tlayout_ms) * 1e-3) / 1e6, 2)},
+        "build_1m": build_1m,
         "trbvh_build": {"ms": round(trbvh_ms, 3),
                         "Mtris_s": round(N_TRIS / (trbvh_ms * 1e-3) / 1e6, 2)},
         "trace": trace,
