@@ -559,7 +559,14 @@ int orc_trbvh_schedule(int treelet, int warp, int32_t* out, int cap)
     int max_rounds = nsub + 1;
     int* fill = (int*)calloc((size_t)max_rounds, sizeof(int));
     int32_t* rounds = (int32_t*)calloc((size_t)max_rounds * warp, sizeof(int32_t));
-    if (!c.deps || !c.expanded || !round_of || !fill || !rounds) return -1;
+    if (!c.deps || !c.expanded || !round_of || !fill || !rounds) {
+        free(c.deps);
+        free(c.expanded);
+        free(round_of);
+        free(fill);
+        free(rounds);
+        return -1;
+    }
     for (int s = 0; s < nsub; ++s) round_of[s] = -1;
     sched_process(&c, nsub - 1, 0);
     int nrounds = 0;
@@ -1310,7 +1317,12 @@ int orc_collapse(const orc_tree* t, float ci, float ct, orc_collapsed* c, float*
     x.cost = (float*)malloc(sizeof(float) * nodes);
     x.flag = (uint8_t*)calloc(nodes, 1);
     x.deliver = (uint8_t*)calloc(nodes, 1);
-    if (!x.cost || !x.flag || !x.deliver) return -1;
+    if (!x.cost || !x.flag || !x.deliver) {
+        free(x.cost);
+        free(x.flag);
+        free(x.deliver);
+        return -1;
+    }
     postorder((orc_tree*)t, &x, cost_visit);
 
     uint32_t position = 0;
